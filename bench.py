@@ -1,0 +1,220 @@
+"""VMC benchmark: walker-steps/s of the Metropolis hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--walkers 8192]
+                    [--system h2o] [--precision mixed] [--no-cpu-baseline]
+
+Workload (one "step" = one all-electron Metropolis move of every walker:
+proposal, psi^2 at the proposal, accept/reject): H2O at equilibrium
+(O-H 0.9572 A, 104.52 deg), cc-pVDZ (25 Cartesian AOs), 5 up + 5 down
+electrons, single determinant x Pade e-e x Pade e-n Jastrow, 8192 walkers
+initialised around the nuclei.  MO coefficients are seeded random orthonormal
+(no SCF program offline) -- the cost of a step does not depend on their values.
+
+Prints one JSON line (rank 0).  Besides the metric it reports the local-energy
+throughput, the local-energy error of the mixed-precision path against the
+FP64 path on the sampled positions, the roofline position of the sweep kernel
+(HIP events on the launch stream) and a CPU baseline: the oracle restatement
+of the reference algorithm on the host cores, on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "VMC walker-steps/s at <=1e-6 Ha/elec error vs FP64; p50/p99 request latency"
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 (vector with packed FP32 / matrix)
+
+
+def algorithmic_cost(mol, wf):
+    """Per walker-step flops and transcendentals of the reference algorithm
+    (F_step / T_step of the benchmark definition), from the real basis."""
+    ne, nat = mol.nelec, mol.natom
+    npr = len(mol.basis.bas_exp)
+    nao = mol.basis.nao
+    nocc = wf.highest_occ_mo
+    nup, ndn = mol.nup, mol.ndown
+    u = 1
+    F = (15 * ne * npr + 2 * (nup + ndn) * nao * nocc + u * (2.0 / 3.0) * (nup ** 3 + ndn ** 3)
+         + 12 * ne * (ne - 1) / 2 + 12 * ne * nat + 25)
+    T = ne * npr + ne * (ne - 1) + 2 * ne * nat + 6 * ne + 2
+    return F, T
+
+
+def build_system(name):
+    from tests import systems
+
+    return {"h2o": systems.h2o, "lih": systems.lih, "h2": systems.h2, "n2": systems.n2,
+            "c6h6": systems.benzene}[name]()
+
+
+def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=150, warm=2):
+    """Oracle restatement of the reference Metropolis step on the host CPU."""
+    import torch
+
+    from oracle import vmc_oracle as O
+    from tests import helpers
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    ee, en = wf_dev._jastrow_terms()
+    orc = O.OracleSlaterJastrow(mol.basis, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown,
+                                jastrow_ee=1.0 if ee is not None else None, jastrow_en=1.0 if en is not None else None)
+    pos = helpers.positions(mol, sample_walkers, seed=3)
+    O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0)
+    t0 = time.perf_counter()
+    O.metropolis(orc.pdf, pos.clone(), sample_steps, step_size=0.2, seed=0)
+    dt = time.perf_counter() - t0
+    return {"value": sample_walkers * sample_steps / dt, "unit": "walker-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": "%d walkers x %d Metropolis steps of the same system, torch CPU float64 (%.1f s)"
+                      % (sample_walkers, sample_steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--walkers", type=int, default=8192)
+    ap.add_argument("--system", default="h2o")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--steps-per-launch", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the VMC hot path on a single MI355X "
+              "(walkers shard trivially across GPUs, see DESIGN.md); --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from qmctorch_amd import _native as N
+    from tests import helpers
+
+    torch.set_default_dtype(torch.float64)
+    mol = build_system(args.system)
+    wf, _ = helpers.make_pair(mol, w_ee=1.0, w_en=1.0, perturb=0.0, precision=args.precision)
+    nw = args.walkers
+    dev = torch.device("cuda")
+    x = helpers.positions(mol, nw, seed=0).to(dev).contiguous()
+    logabs = torch.empty(nw, dtype=torch.float64, device=dev)
+    sign = torch.empty(nw, dtype=torch.float64, device=dev)
+    nacc = torch.zeros(nw, dtype=torch.int32, device=dev)
+    plan = wf.device_plan()
+    prm, keep = wf._params(args.precision)
+    prec = N.QMC_PREC_MIXED if args.precision == "mixed" else N.QMC_PREC_FP64
+    lib = N.load()
+    wk = N.QmcWalkers()
+    wk.pos, wk.logabs, wk.sign, wk.naccept, wk.walker_id = x.data_ptr(), logabs.data_ptr(), sign.data_ptr(), \
+        nacc.data_ptr(), 0
+    sw = N.QmcSweepDesc()
+    sw.sigma2 = 0.2 / (2 * math.sqrt(2 * math.log(2.0)))
+    sw.seed = 12345
+    sw.keep = 0
+    sw.keep_every = 1
+    sw.step_size = 0.2
+    stream = N.current_stream()
+    spl = args.steps_per_launch
+
+    def run(step0, nstep, init, events=None):
+        s = step0
+        first = init
+        while s < step0 + nstep:
+            sw.step0 = s
+            sw.nstep = min(spl, step0 + nstep - s)
+            sw.init_logpsi = 1 if first else 0
+            if events is not None:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            N.check(lib.qmc_metropolis_sweep(plan.handle, prm, prec, wk, nw, sw, stream))
+            if events is not None:
+                e1.record()
+                events.append((e0, e1, sw.nstep))
+            first = False
+            s += sw.nstep
+
+    run(0, max(args.warmup, 1), True)
+    torch.cuda.synchronize()
+    nacc.zero_()
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(args.warmup, args.steps, False, events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = nw * args.steps / elapsed
+    acc = float(nacc.sum().item()) / (nw * args.steps)
+    full = [(a.elapsed_time(b) * 1e-3, n) for a, b, n in events if n == spl]
+    launch_s = float(np.mean([t for t, _ in full])) if full else elapsed / max(len(events), 1)
+    steps_per_launch = spl if full else args.steps
+
+    F, T = algorithmic_cost(mol, wf)
+    flops_launch = (F + 4 * T) * nw * steps_per_launch
+    achieved = flops_launch / launch_s / 1e12
+
+    # local energy throughput and mixed-vs-fp64 error on the sampled positions
+    pos = x.clone()
+    wf.local_energy_parts(pos, args.precision)
+    torch.cuda.synchronize()
+    reps = 5
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        parts = wf.local_energy_parts(pos, args.precision)
+    torch.cuda.synchronize()
+    el_rate = reps * nw / (time.perf_counter() - t1)
+    e64 = wf.local_energy_parts(pos, "fp64")["eloc"]
+    emx = wf.local_energy_parts(pos, "mixed")["eloc"]
+    d = ((emx - e64).abs() / mol.nelec)
+    err = {"max": float(d.max()), "median": float(d.median()), "mean": float(d.mean()),
+           "bias_of_mean": float((emx.mean() - e64.mean()).abs() / mol.nelec)}
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s Metropolis all-electron sweep, %d walkers, step 0.2 bohr, SJ ee+en Pade, %s"
+                        % (args.system, nw, args.precision),
+            "system": args.system, "walkers": nw, "nelec": mol.nelec, "nao": mol.basis.nao,
+            "nprim": len(mol.basis.bas_exp), "steps_per_launch": steps_per_launch,
+            "parallelism": "single GPU",
+        },
+        "acceptance": acc,
+        "local_energy_per_s": el_rate,
+        "eloc_error_per_electron_vs_fp64": err,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "sweep_kernel",
+            "launch_ms": launch_s * 1e3,
+            "flops_per_walker_step": F + 4 * T,
+        },
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(mol, wf)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

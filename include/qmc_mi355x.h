@@ -1,0 +1,205 @@
+/*
+ * qmc_mi355x.h -- C ABI of libqmcmi355x.so, the MI355X (gfx950) kernels of the
+ * QMCTorch VMC hot path.
+ *
+ * The reference (QMCTorch 0.4.0) is pure Python; its "operator API" for this
+ * path is the Python module surface listed next to each entry point.  The
+ * Python package qmctorch_amd keeps that surface and calls these functions
+ * through ctypes; every pointer argument is a device pointer (torch caching
+ * allocator), sizes are element counts, every function returns 0 on success
+ * or a nonzero status whose text is available from qmc_last_error().  No entry
+ * point synchronises the stream it is given.
+ */
+#ifndef QMC_MI355X_H
+#define QMC_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* qmc_stream_t; /* hipStream_t of the caller (0 = default stream) */
+
+enum { QMC_OK = 0, QMC_ERR_ARG = 1, QMC_ERR_HIP = 2, QMC_ERR_UNSUPPORTED = 3 };
+
+enum { QMC_RADIAL_GTO_PURE = 0, QMC_RADIAL_STO = 1 };
+
+enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 }; /* MIXED: FP32 AO/MO/Jastrow, FP64 Slater/energies */
+
+/* Maximum number of primitives sharing one exponent group, and of unique
+ * configurations per spin handled by the kernels. */
+#define QMC_MAXP 12
+#define QMC_MAXCONF 8
+
+/*
+ * Static description of one Slater-Jastrow wavefunction (host pointers; the
+ * plan copies them to the device).  Built by qmctorch_amd.wavefunction.plan
+ * from the same basis namespace the reference consumes
+ * (qmctorch/scf/calculator/pyscf.py:104-252, molecule.py:305-350):
+ *
+ *  exponent groups  -- primitives of one atom that share an exponent list;
+ *                      group g has grp_nprim[g] exponents taken from bas_exp at
+ *                      indices grp_prim_src[grp_prim0[g] + p] and grp_nao[g]
+ *                      AO entries starting at grp_ao0[g];
+ *  AO entries       -- gao_mu (AO index), gao_k (kx | ky<<4 | kz<<8) and
+ *                      QMC_MAXP coefficients gao_K (norm_cst * bas_coeffs of
+ *                      atomic_orbitals.py:236-249,654-669) and r powers gao_n
+ *                      (STO only);
+ *  configurations   -- unique spin-up/down orbital lists and the CI map of
+ *                      slater_pooling.py:96-111 / orbital_projector.py:29-94.
+ */
+typedef struct QmcSystemDesc {
+    int32_t nelec, nup, ndown, natom;
+    int32_t nao;
+    int32_t nprim_ref;   /* length of bas_exp (reference primitive count Np) */
+    int32_t radial_type; /* QMC_RADIAL_* */
+    int32_t ngroup;
+    const int32_t* grp_atom;     /* [ngroup] */
+    const int32_t* grp_nprim;    /* [ngroup] <= QMC_MAXP */
+    const int32_t* grp_prim0;    /* [ngroup] offset into grp_prim_src */
+    const int32_t* grp_prim_src; /* [sum nprim] index into bas_exp */
+    const int32_t* grp_ao0;      /* [ngroup] offset into AO entries */
+    const int32_t* grp_nao;      /* [ngroup] */
+    int32_t nentry;              /* number of AO entries */
+    const int32_t* gao_mu;       /* [nentry] */
+    const int32_t* gao_k;        /* [nentry] */
+    const double* gao_K;         /* [nentry * QMC_MAXP] */
+    const double* gao_n;         /* [nentry * QMC_MAXP] (STO r powers, else 0) */
+    const int32_t* gao_src;      /* [nentry * QMC_MAXP] index into bas_exp of each term */
+    const double* atom_z;        /* [natom] nuclear charges */
+    const double* jastrow_atoms; /* [natom*3] e-n Jastrow centres (mol.atom_coords) */
+    int32_t nmo;                 /* MO columns used (highest occupied index + 1) */
+    int32_t nci;
+    int32_t nconf_up, nconf_dn;  /* unique configurations per spin (<= QMC_MAXCONF) */
+    const int32_t* conf_up;      /* [nconf_up * nup] MO columns */
+    const int32_t* conf_dn;      /* [nconf_dn * ndown] */
+    const int32_t* ci_up;        /* [nci] unique up configuration of CI term n */
+    const int32_t* ci_dn;        /* [nci] */
+    int32_t use_jastrow_ee, use_jastrow_en;
+    double vnn;                  /* nuclear repulsion (wf_base.py:97-116) */
+} QmcSystemDesc;
+
+typedef struct QmcPlan QmcPlan;
+
+/*
+ * Trainable parameters, read on every call (device pointers, float64), so
+ * optimizer updates made in place are seen without re-planning:
+ *   bas_exp     ao.bas_exp            [nprim_ref]   (atomic_orbitals.py:53-55)
+ *   atom_coords ao.atom_coords        [natom*3]     (atomic_orbitals.py:33-37)
+ *   mo_w        mo_scf * mo_modifier  [nao*nmo]     float32 (MIXED) or float64,
+ *               row-major, first nmo columns (molecular_orbitals.py:79-95)
+ *   ci          fc.weight             [nci]         (slater_jastrow.py:179-191)
+ *   w_ee, w_en  Pade weights          [1] or NULL   (pade_jastrow_kernel.py:28)
+ */
+typedef struct QmcParams {
+    const double* bas_exp;
+    const double* atom_coords;
+    const void* mo_w;
+    const double* ci;
+    const double* w_ee;
+    const double* w_en;
+} QmcParams;
+
+/* Outputs of qmc_wf_eval; any pointer may be NULL.  All float64, [nw]. */
+typedef struct QmcWfOut {
+    double* psi;     /* J * fc(dets)                 slater_jastrow.py:243-286 */
+    double* logabs;  /* log|psi| */
+    double* sign;    /* sign(psi) */
+    double* kinetic; /* kinetic_energy_jacobi        slater_jastrow.py:312-344 */
+    double* eloc;    /* local_energy                 wf_base.py:184-215 */
+    double* epot;    /* V_en + V_ee + V_nn           wf_base.py:49-116 */
+} QmcWfOut;
+
+enum { QMC_EVAL_PSI = 0, QMC_EVAL_ELOC = 1 };
+
+/* Walker state of a Metropolis chain (device pointers). */
+typedef struct QmcWalkers {
+    double* pos;      /* [nw * 3*nelec], updated in place */
+    double* logabs;   /* [nw] log|psi| of pos, updated in place */
+    double* sign;     /* [nw] */
+    int32_t* naccept; /* [nw] accepted moves, incremented */
+    const int64_t* walker_id; /* [nw] global walker ids keying the RNG, or NULL (= index) */
+} QmcWalkers;
+
+/*
+ * Metropolis sweep (sampler/metropolis.py:85-298): all-electron Gaussian moves
+ * with covariance sigma2 * I per electron (metropolis.py:206-212: sigma2 =
+ * step/(2 sqrt(2 ln 2))), acceptance min(1, psi'^2/psi^2) >= U.  Steps
+ * step0 .. step0+nstep-1 of the chain are performed; random numbers come from
+ * Philox4x32-10 keyed by seed and counted by (walker id, step, electron), so a
+ * walker's trajectory does not depend on how walkers are batched.  Positions
+ * of steps s with s >= keep_from and (s - keep_from) % keep_every == 0 are
+ * written to keep[(s - keep_from)/keep_every][walker] when keep != NULL.
+ */
+typedef struct QmcSweepDesc {
+    int64_t step0;
+    int32_t nstep;
+    int32_t init_logpsi; /* 1: evaluate logabs/sign of pos first */
+    double sigma2;
+    uint64_t seed;
+    int64_t keep_from;
+    int32_t keep_every;
+    int32_t uniform_move; /* 1: uniform cube proposal step*(2U-1) (metropolis.py:263-269) */
+    double step_size;
+    double* keep;        /* [nkeep * nw * 3*nelec] or NULL */
+} QmcSweepDesc;
+
+/* Plan lifetime.  Replaces SlaterJastrow.__init__ (slater_jastrow.py:31-117)
+ * bookkeeping: device copies of the basis, configuration and Jastrow tables. */
+int qmc_plan_create(const QmcSystemDesc* desc, QmcPlan** plan);
+int qmc_plan_destroy(QmcPlan* plan);
+
+/* psi / log|psi| / kinetic / local energy of nw walkers.
+ * Replaces SlaterJastrow.forward (slater_jastrow.py:243-286), WaveFunction.pdf
+ * (wf_base.py:241-247), SlaterJastrow.kinetic_energy_jacobi
+ * (slater_jastrow.py:312-344) and WaveFunction.local_energy (wf_base.py:184-215). */
+int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision, int32_t mode,
+                const double* pos, int64_t nw, QmcWfOut* out, qmc_stream_t stream);
+
+/* Atomic orbitals and their derivatives at nw walkers, reference layouts:
+ * ao [nw][nelec][nao]; dao [nw][nelec][nao][3]; d2ao [nw][nelec][nao] (float64).
+ * Replaces AtomicOrbitals.forward(pos, derivative=[0] or [0,1,2], sum_grad=False)
+ * (atomic_orbitals.py:131-218, 578-609).  dao/d2ao may be NULL. */
+int qmc_ao_eval(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                double* ao, double* dao, double* d2ao, qmc_stream_t stream);
+
+/* Metropolis sweeps.  Replaces the step loop of Metropolis.__call__
+ * (metropolis.py:134-165) for move={'type':'all-elec'}. */
+int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precision,
+                         QmcWalkers* walkers, int64_t nw, const QmcSweepDesc* sweep, qmc_stream_t stream);
+
+/* Adjoint of log|psi| with respect to the parameters, as per-walker rows
+ * (AO values are recomputed here, not stored by the forward pass):
+ *   ao_rows    [nw*nelec][nao]  AO values of every electron (reference AO layout)
+ *   gamma_rows [nw*nelec][nmo]  d log|psi| / d MO[i,m] = sum_c omega_c (A_c^-1)[k(m), i]
+ *   dci        [nw][nci]        d log|psi| / d c_n = D_n / sum_k c_k D_k
+ *   djas       [nw][2]          d log|psi| / d (w_ee, w_en)
+ *   dexp_rows  [nw*nelec][nprim_ref] d log|psi| / d bas_exp per electron, or NULL
+ *              (norm_cst is a constant of the AO layer, atomic_orbitals.py:90-94)
+ * so that for walker weights g: dL/dW = ao_rows^T (g * gamma_rows) (one GEMM),
+ * dL/dc = g^T dci, dL/dw = g^T djas, dL/dexp = g_rows^T dexp_rows.
+ * FP64 throughout; mo_w must be float64.
+ * Replaces autograd through SlaterJastrow.forward in Solver.evaluate_grad_manual
+ * (solver.py:372-431). */
+int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                         double* ao_rows, double* gamma_rows, double* dci, double* djas, double* dexp_rows,
+                         qmc_stream_t stream);
+
+/* Philox4x32-10 of the sweep kernel applied to n (counter[4], key[2]) tuples
+ * (device pointers: in [n*6] uint32, out [n*4] uint32).  No counterpart in the
+ * reference (whose proposals come from torch's host generator); exposed so the
+ * device generator can be checked against the published known-answer vectors. */
+int qmc_philox4x32_10(const uint32_t* in, uint32_t* out, int64_t n, qmc_stream_t stream);
+
+/* Text of the last error on the calling thread. */
+const char* qmc_last_error(void);
+
+/* Library version string. */
+const char* qmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMC_MI355X_H */

@@ -1,0 +1,75 @@
+"""Build libqmcmi355x.so (HIP, gfx950) in-tree.
+
+    python -m qmctorch_amd.csrc.build [--verbose] [--resource-usage]
+
+Each kernel family is its own translation unit, compiled in parallel and
+linked into qmctorch_amd/libqmcmi355x.so (next to the package, so the library
+travels with the source tree).  Packed FP32 instructions are disabled
+(-packed-fp32-ops); the compiler would otherwise form v_pk_* from float code.
+"""
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "libqmcmi355x.so")
+SOURCES = ["qmc_abi.hip", "qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip",
+           "qmc_grad.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-mcode-object-version=5",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+    "-I", HERE,
+]
+
+
+def _obj_dir() -> str:
+    d = os.path.join(HERE, "_build")
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def compile_one(src: str, extra=()) -> subprocess.CompletedProcess:
+    obj = os.path.join(_obj_dir(), src.replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) -> str:
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 4, 16)
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda s: compile_one(s, extra), SOURCES))
+    for src, r in zip(SOURCES, results):
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed on %s" % src)
+        if verbose or resource_usage:
+            sys.stderr.write(r.stderr)
+    objs = [os.path.join(_obj_dir(), s.replace(".hip", ".o")) for s in SOURCES]
+    tmp = LIB + ".tmp%d" % os.getpid()
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link of libqmcmi355x.so failed")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--resource-usage", action="store_true")
+    ap.add_argument("-j", type=int, default=0)
+    a = ap.parse_args()
+    print(build(a.verbose, a.resource_usage, a.j))

@@ -1,0 +1,259 @@
+// libqmcmi355x: C ABI of the QMCTorch VMC hot path on MI355X (gfx950).
+// Plan lifetime, argument checks and dispatch to the kernel families
+// (qmc_wf_*.hip, qmc_sweep_*.hip, qmc_grad.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "qmc_launch.h"
+
+using namespace qmc;
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define QMC_HIP(call)                                                                                    \
+    do {                                                                                                 \
+        hipError_t _e = (call);                                                                          \
+        if (_e != hipSuccess) return fail(QMC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+const char* kNoBucket = "more than 24 occupied orbitals per spin";
+
+__global__ void philox_kernel(const uint32_t* in, uint32_t* out, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* c = in + 6 * i;
+    const u4 r = philox4x32_10(u4{c[0], c[1], c[2], c[3]}, c[4], c[5]);
+    out[4 * i + 0] = r.x;
+    out[4 * i + 1] = r.y;
+    out[4 * i + 2] = r.z;
+    out[4 * i + 3] = r.w;
+}
+}  // namespace
+
+// =========================================================================== plan
+struct QmcPlan {
+    Tables t;
+    std::vector<void*> allocs;
+    ~QmcPlan() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    template <typename X>
+    const X* upload(const X* host, size_t n) {
+        if (n == 0 || host == nullptr) return nullptr;
+        void* d = nullptr;
+        if (hipMalloc(&d, n * sizeof(X)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
+        allocs.push_back(d);
+        if (hipMemcpy(d, host, n * sizeof(X), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::runtime_error("hipMemcpy failed");
+        return (const X*)d;
+    }
+};
+
+extern "C" {
+
+const char* qmc_last_error(void) { return g_last_error.c_str(); }
+
+const char* qmc_version(void) { return "libqmcmi355x 0.1 (gfx950)"; }
+
+int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
+    if (!d || !out) return fail(QMC_ERR_ARG, "null argument");
+    if (d->nelec < 1 || d->nelec > 64) return fail(QMC_ERR_UNSUPPORTED, "nelec must be in [1, 64]");
+    if (d->nup + d->ndown != d->nelec || d->ndown > d->nup) return fail(QMC_ERR_ARG, "inconsistent spin counts");
+    if (d->radial_type != QMC_RADIAL_GTO_PURE && d->radial_type != QMC_RADIAL_STO)
+        return fail(QMC_ERR_UNSUPPORTED, "radial type must be gto_pure or sto");
+    if (d->nconf_up < 1 || d->nconf_up > QMC_MAXCONF || d->nconf_dn < 1 || d->nconf_dn > QMC_MAXCONF)
+        return fail(QMC_ERR_UNSUPPORTED, "unique configurations per spin must be in [1, 8]");
+    if (d->nmo < d->nup) return fail(QMC_ERR_ARG, "nmo smaller than nup");
+    for (int g = 0; g < d->ngroup; ++g)
+        if (d->grp_nprim[g] > QMC_MAXP || d->grp_nprim[g] < 1) return fail(QMC_ERR_UNSUPPORTED, "group too large");
+    QmcPlan* p = nullptr;
+    try {
+        p = new QmcPlan();
+        Tables& t = p->t;
+        std::memset(&t, 0, sizeof(t));
+        t.nelec = d->nelec;
+        t.nup = d->nup;
+        t.ndown = d->ndown;
+        t.natom = d->natom;
+        t.nao = d->nao;
+        t.nprim_ref = d->nprim_ref;
+        t.radial = d->radial_type;
+        t.ngroup = d->ngroup;
+        t.nentry = d->nentry;
+        t.nmo = d->nmo;
+        t.nci = d->nci;
+        t.nconf_up = d->nconf_up;
+        t.nconf_dn = d->nconf_dn;
+        t.nconf_max = d->nconf_up > d->nconf_dn ? d->nconf_up : d->nconf_dn;
+        t.nsig_max = d->nup;
+        t.use_ee = d->use_jastrow_ee;
+        t.use_en = d->use_jastrow_en;
+        t.vnn = d->vnn;
+        // identity configuration: a single configuration per spin with columns 0..n-1
+        bool ident = d->nconf_up == 1 && d->nconf_dn == 1 && d->nci == 1;
+        for (int j = 0; ident && j < d->nup; ++j) ident = d->conf_up[j] == j;
+        for (int j = 0; ident && j < d->ndown; ++j) ident = d->conf_dn[j] == j;
+        t.identity_conf = ident ? 1 : 0;
+        int nprim_tot = 0;
+        for (int g = 0; g < d->ngroup; ++g) nprim_tot = std::max(nprim_tot, d->grp_prim0[g] + d->grp_nprim[g]);
+        t.grp_atom = p->upload(d->grp_atom, d->ngroup);
+        t.grp_nprim = p->upload(d->grp_nprim, d->ngroup);
+        t.grp_prim0 = p->upload(d->grp_prim0, d->ngroup);
+        t.grp_prim_src = p->upload(d->grp_prim_src, nprim_tot);
+        t.grp_ao0 = p->upload(d->grp_ao0, d->ngroup);
+        t.grp_nao = p->upload(d->grp_nao, d->ngroup);
+        t.gao_mu = p->upload(d->gao_mu, d->nentry);
+        t.gao_k = p->upload(d->gao_k, d->nentry);
+        t.gao_K = p->upload(d->gao_K, (size_t)d->nentry * QMC_MAXP);
+        t.gao_n = p->upload(d->gao_n, (size_t)d->nentry * QMC_MAXP);
+        t.gao_src = p->upload(d->gao_src, (size_t)d->nentry * QMC_MAXP);
+        t.atom_z = p->upload(d->atom_z, d->natom);
+        t.jas_atoms = p->upload(d->jastrow_atoms, (size_t)d->natom * 3);
+        t.conf_up = p->upload(d->conf_up, (size_t)d->nconf_up * d->nup);
+        t.conf_dn = p->upload(d->conf_dn, (size_t)d->nconf_dn * (d->ndown > 0 ? d->ndown : 1));
+        t.ci_up = p->upload(d->ci_up, d->nci);
+        t.ci_dn = p->upload(d->ci_dn, d->nci);
+        if (!t.conf_dn) t.conf_dn = t.conf_up;
+    } catch (const std::exception& ex) {
+        delete p;
+        return fail(QMC_ERR_HIP, ex.what());
+    }
+    *out = p;
+    return QMC_OK;
+}
+
+int qmc_plan_destroy(QmcPlan* plan) {
+    delete plan;
+    return QMC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_common(const QmcPlan* plan, const QmcParams* prm, int64_t nw) {
+    if (!plan || !prm) return fail(QMC_ERR_ARG, "null plan or params");
+    if (nw < 0) return fail(QMC_ERR_ARG, "negative walker count");
+    if (!prm->bas_exp || !prm->atom_coords || !prm->mo_w || !prm->ci)
+        return fail(QMC_ERR_ARG, "missing parameter pointer");
+    if (plan->t.use_ee && !prm->w_ee) return fail(QMC_ERR_ARG, "e-e Jastrow weight missing");
+    if (plan->t.use_en && !prm->w_en) return fail(QMC_ERR_ARG, "e-n Jastrow weight missing");
+    return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision, int32_t mode, const double* pos,
+                int64_t nw, QmcWfOut* out, qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!pos || !out) return fail(QMC_ERR_ARG, "null pos/out");
+    if (nw == 0) return QMC_OK;
+    KArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.out = *out;
+    hipStream_t s = (hipStream_t)stream;
+    const bool sto = plan->t.radial == QMC_RADIAL_STO;
+    (void)sto;
+    rc = precision == QMC_PREC_FP64 ? launch_wf_f64(a, mode, s) : launch_wf_f32(a, mode, s);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precision, QmcWalkers* wk, int64_t nw,
+                         const QmcSweepDesc* sw, qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!wk || !sw || !wk->pos || !wk->logabs || !wk->sign) return fail(QMC_ERR_ARG, "null walker state");
+    if (sw->keep && sw->keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
+    if (nw == 0 || sw->nstep <= 0) return QMC_OK;
+    SArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.wk = *wk;
+    a.sw = *sw;
+    a.nw = nw;
+    hipStream_t s = (hipStream_t)stream;
+    const bool sto = plan->t.radial == QMC_RADIAL_STO;
+    (void)sto;
+    rc = precision == QMC_PREC_FP64 ? launch_sweep_f64(a, s) : launch_sweep_f32(a, s);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_ao_eval(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw, double* ao,
+                double* dao, double* d2ao, qmc_stream_t stream) {
+    if (!plan || !params || !params->bas_exp || !params->atom_coords) return fail(QMC_ERR_ARG, "null argument");
+    if (!pos || !ao) return fail(QMC_ERR_ARG, "null pos/ao");
+    if (nw == 0) return QMC_OK;
+    AArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.ao = ao;
+    a.dao = dao;
+    a.d2ao = d2ao;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)nw * plan->t.nelec * plan->t.nao;
+    // the kernel accumulates (AO entries split over groups), so clear first
+    fill_zero(ao, n, s);
+    if (dao) fill_zero(dao, 3 * n, s);
+    if (d2ao) fill_zero(d2ao, n, s);
+    launch_ao(a, dao || d2ao, s);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                         double* ao_rows, double* gamma_rows, double* dci, double* djas, double* dexp_rows,
+                         qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!pos || !ao_rows || !gamma_rows || !dci || !djas) return fail(QMC_ERR_ARG, "null output");
+    if (nw == 0) return QMC_OK;
+    GArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.ao_rows = ao_rows;
+    a.gamma_rows = gamma_rows;
+    a.dci = dci;
+    a.djas = djas;
+    a.dexp_rows = dexp_rows;
+    hipStream_t s = (hipStream_t)stream;
+    fill_zero(ao_rows, (size_t)nw * plan->t.nelec * plan->t.nao, s);
+    if (dexp_rows) fill_zero(dexp_rows, (size_t)nw * plan->t.nelec * plan->t.nprim_ref, s);
+    if (launch_grad(a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_philox4x32_10(const uint32_t* in, uint32_t* out, int64_t n, qmc_stream_t stream) {
+    if (!in || !out || n < 0) return fail(QMC_ERR_ARG, "bad philox arguments");
+    if (n == 0) return QMC_OK;
+    hipLaunchKernelGGL(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, out, n);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,292 @@
+// Device building blocks of the VMC kernels (gfx950, wave64).
+//
+// Lane layout used by every kernel: a wavefront holds G = 64 / nelec walkers,
+// lane l = wl * nelec + e serves electron e of walker wl.  Basis, MO and CI
+// tables are identical for all lanes of a launch (scalar-cache loads); each
+// lane keeps its electron's MO row in registers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qmc_mi355x.h"
+
+namespace qmc {
+
+struct Tables {
+    int nelec, nup, ndown, natom, nao, nprim_ref, radial;
+    int ngroup, nentry, nmo, nci, nconf_up, nconf_dn, nconf_max, nsig_max;
+    int use_ee, use_en, identity_conf;
+    double vnn;
+    const int* grp_atom;
+    const int* grp_nprim;
+    const int* grp_prim0;
+    const int* grp_prim_src;
+    const int* grp_ao0;
+    const int* grp_nao;
+    const int* gao_mu;
+    const int* gao_k;
+    const double* gao_K;
+    const double* gao_n;
+    const int* gao_src;
+    const double* atom_z;
+    const double* jas_atoms;
+    const int* conf_up;
+    const int* conf_dn;
+    const int* ci_up;
+    const int* ci_dn;
+};
+
+// ------------------------------------------------------------------ math
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+
+template <typename T>
+__device__ __forceinline__ T ipow(T x, int k) {
+    // k in 0..3 (Cartesian exponents / STO r powers of the supported bases)
+    T x2 = x * x;
+    return k == 0 ? T(1) : (k == 1 ? x : (k == 2 ? x2 : x2 * x));
+}
+
+// Cartesian polynomial x^kx y^ky z^kz with gradient and Laplacian
+// (spherical_harmonics.py:102-199, Cartesian branch).
+template <typename T, bool DERIV>
+__device__ __forceinline__ void cart_poly(T x, T y, T z, int kx, int ky, int kz, T& Y, T& Yx, T& Yy, T& Yz,
+                                          T& Ylap) {
+    T px = ipow(x, kx), py = ipow(y, ky), pz = ipow(z, kz);
+    Y = px * py * pz;
+    if (DERIV) {
+        T dx = kx > 0 ? T(kx) * ipow(x, kx - 1) : T(0);
+        T dy = ky > 0 ? T(ky) * ipow(y, ky - 1) : T(0);
+        T dz = kz > 0 ? T(kz) * ipow(z, kz - 1) : T(0);
+        Yx = dx * py * pz;
+        Yy = px * dy * pz;
+        Yz = px * py * dz;
+        T d2x = kx > 1 ? T(kx * (kx - 1)) * ipow(x, kx - 2) : T(0);
+        T d2y = ky > 1 ? T(ky * (ky - 1)) * ipow(y, ky - 2) : T(0);
+        T d2z = kz > 1 ? T(kz * (kz - 1)) * ipow(z, kz - 2) : T(0);
+        Ylap = d2x * py * pz + px * d2y * pz + px * py * d2z;
+    }
+}
+
+// ------------------------------------------------------------ AO -> MO row
+// Evaluates the AOs of one electron group by group and contracts them into
+// the MO row on the fly (the AO row never leaves registers).
+//
+//   GTO (radial_functions.py:241-320): with S_k = sum_p K_p a_p^k exp(-a_p r^2)
+//     AO = Y S0,  grad AO = S0 grad Y - 2 S1 Y r,
+//     lap AO = Y (4 r^2 S2 - (6 + 4L) S1) + S0 lap Y          (L = kx+ky+kz)
+//   STO (radial_functions.py:6-123): R_p = r^n exp(-a r), q = n/r^2 - a/r,
+//     AO = Y sum K R,  grad AO = (sum K R) grad Y + Y (sum K R q) r,
+//     lap AO = Y sum K R (n(n+1)/r^2 - 2a(n+1)/r + a^2 + 2Lq) + (sum K R) lap Y
+//
+// AOSINK(mu, ao, gx, gy, gz, lap) is called once per AO entry; entries of one
+// AO that were split across groups are delivered separately (all consumers
+// are linear in the AO).
+template <typename T, int RAD, bool DERIV, typename Sink>
+__device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict__ bas_exp,
+                                       const double* __restrict__ atom_xyz, double ex, double ey, double ez,
+                                       Sink&& sink) {
+    int cur_atom = -1;
+    T dx = 0, dy = 0, dz = 0, r2 = 0, r = 0, ir = 0;
+    for (int g = 0; g < t.ngroup; ++g) {
+        const int at = t.grp_atom[g];
+        if (at != cur_atom) {
+            cur_atom = at;
+            dx = (T)(ex - atom_xyz[3 * at + 0]);
+            dy = (T)(ey - atom_xyz[3 * at + 1]);
+            dz = (T)(ez - atom_xyz[3 * at + 2]);
+            r2 = dx * dx + dy * dy + dz * dz;
+            if (RAD == QMC_RADIAL_STO) {
+                r = fsqrt(r2);
+                ir = T(1) / r;
+            }
+        }
+        const int np = t.grp_nprim[g];
+        const int p0 = t.grp_prim0[g];
+        T E[QMC_MAXP], A[QMC_MAXP];
+#pragma unroll
+        for (int p = 0; p < QMC_MAXP; ++p) {
+            if (p < np) {
+                const T a = (T)bas_exp[t.grp_prim_src[p0 + p]];
+                A[p] = a;
+                E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
+            } else {
+                A[p] = T(0);
+                E[p] = T(0);
+            }
+        }
+        const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
+        for (int i = i0; i < i1; ++i) {
+            const int kk = t.gao_k[i];
+            const int kx = kk & 15, ky = (kk >> 4) & 15, kz = (kk >> 8) & 15;
+            T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
+            cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
+            const double* Kp = t.gao_K + (size_t)i * QMC_MAXP;
+            T s0 = 0, s1 = 0, s2 = 0;
+            if (RAD == QMC_RADIAL_GTO_PURE) {
+#pragma unroll
+                for (int p = 0; p < QMC_MAXP; ++p) {
+                    if (p < np) {
+                        const T ke = (T)Kp[p] * E[p];
+                        s0 += ke;
+                        if (DERIV) {
+                            s1 += ke * A[p];
+                            s2 += ke * A[p] * A[p];
+                        }
+                    }
+                }
+                const T ao = Y * s0;
+                if (DERIV) {
+                    const T L = T(kx + ky + kz);
+                    const T c = T(-2) * s1 * Y;
+                    sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
+                         Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
+                } else {
+                    sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+                }
+            } else {
+                const double* Np = t.gao_n + (size_t)i * QMC_MAXP;
+                const T L = T(kx + ky + kz);
+#pragma unroll
+                for (int p = 0; p < QMC_MAXP; ++p) {
+                    if (p < np) {
+                        const T n = (T)Np[p];
+                        const int ni = (int)Np[p];
+                        const T kr = (T)Kp[p] * ipow(r, ni) * E[p];
+                        s0 += kr;
+                        if (DERIV) {
+                            const T a = A[p];
+                            const T q = n * ir * ir - a * ir;
+                            s1 += kr * q;
+                            s2 += kr * (n * (n + T(1)) * ir * ir - T(2) * a * (n + T(1)) * ir + a * a + T(2) * L * q);
+                        }
+                    }
+                }
+                const T ao = Y * s0;
+                if (DERIV) {
+                    const T c = Y * s1;
+                    sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
+                } else {
+                    sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------- lane-row elimination
+// Gauss-Jordan elimination with partial pivoting on the n x n matrix whose
+// row r lives in lane gbase + r (r < n), in FP64.  nmax (wave-uniform) is the
+// largest n over the groups of the wave; lanes whose group is smaller simply
+// idle through the extra columns.  On exit every lane of the group holds
+// log|det| and sign(det); with WITH_B the augmented block b (n columns) is
+// reduced to A^-1 B and the lane that pivoted column k returns in trace_part
+// its entry (A^-1 B)_kk, otherwise 0.  Pivot choice: largest |a_rk| among the
+// rows not used yet, first one on ties (LAPACK getrf's rule).
+template <int NM, bool WITH_B>
+__device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[NM], int n, int nmax, int myrow,
+                                                  int gbase, uint64_t gmask, double& logdet, double& sgn,
+                                                  double& trace_part, int& my_col) {
+    bool done = false;
+    my_col = -1;
+    logdet = 0.0;
+    sgn = 1.0;
+    int inversions = 0;
+    bool singular = false;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        if (k < nmax) {
+            const bool colk = k < n;
+            const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+            double best = -1.0;
+            int br = 0;
+            for (int s = 0; s < nmax; ++s) {
+                const double c = __shfl(cand, gbase + (s < n ? s : 0));
+                if (s < n && c > best) {
+                    best = c;
+                    br = s;
+                }
+            }
+            const int plane = gbase + br;
+            // rows not yet used with a smaller index than the pivot row
+            const uint64_t bal = __ballot(!done && myrow < br);
+            if (colk) inversions += __popcll(bal & gmask);
+            const double piv = __shfl(a[k], plane);
+            const bool is_piv = colk && (myrow == br);
+            if (colk) {
+                singular = singular || (piv == 0.0);
+                logdet += log(fabs(piv));
+                sgn *= (piv < 0.0) ? -1.0 : 1.0;
+            }
+            const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+            const double f = (colk && !is_piv && (WITH_B || !done)) ? a[k] * ip : 0.0;
+#pragma unroll
+            for (int j = k + 1; j < NM; ++j) {
+                if (j < nmax) {
+                    const double pj = __shfl(a[j], plane);
+                    a[j] = is_piv ? pj * ip : a[j] - f * pj;
+                }
+            }
+            if (WITH_B) {
+#pragma unroll
+                for (int j = 0; j < NM; ++j) {
+                    if (j < nmax) {
+                        const double pj = __shfl(b[j], plane);
+                        b[j] = is_piv ? pj * ip : b[j] - f * pj;
+                    }
+                }
+            }
+            if (is_piv) {
+                done = true;
+                my_col = k;
+            }
+        }
+    }
+    if (inversions & 1) sgn = -sgn;
+    if (singular) logdet = -INFINITY;
+    trace_part = 0.0;
+    if (WITH_B) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            if (j == my_col) trace_part = b[j];
+    }
+}
+
+// Sum of v over the lanes [base, base+cnt) (result in every lane; cnt uniform bound).
+__device__ __forceinline__ double seg_sum(double v, int base, int cnt, int cnt_max) {
+    double s = 0.0;
+    for (int t = 0; t < cnt_max; ++t) {
+        const double x = __shfl(v, base + (t < cnt ? t : 0));
+        if (t < cnt) s += x;
+    }
+    return s;
+}
+
+// ------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11.  Round constants and Weyl key increments from that paper.
+struct u4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+// (0,1] and [0,1) uniforms from 32 random bits
+__device__ __forceinline__ double u01_open0(uint32_t v) { return ((double)v + 1.0) * 2.3283064365386963e-10; }
+__device__ __forceinline__ double u01_closed0(uint32_t v) { return (double)v * 2.3283064365386963e-10; }
+
+}  // namespace qmc

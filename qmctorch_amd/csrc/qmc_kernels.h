@@ -1,0 +1,703 @@
+// Kernel templates of libqmcmi355x (QMCTorch VMC hot path on MI355X, gfx950).
+//
+// Kernels (all use the lane layout of qmc_device.h: one lane per electron,
+// 64/nelec walkers per wavefront, 4 wavefronts per block):
+//   wf_kernel      psi, log|psi|, kinetic (Jacobi), local energy
+//   sweep_kernel   persistent multi-step Metropolis with on-device Philox
+//   ao_kernel      AO values / gradients / Laplacians in the reference layout
+//   grad_kernel    rows for the adjoint of log|psi| w.r.t. the parameters
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "qmc_device.h"
+
+namespace qmc {
+
+constexpr int kBlock = 256;
+
+struct WalkerLane {
+    int lane, wave, G, wl, wbase, e, w, wclamp, spin, myrow, gbase, n_sig;
+    bool lane_real, valid;
+    uint64_t gmask;
+};
+
+__device__ __forceinline__ WalkerLane lane_layout(const Tables& t, int64_t nw) {
+    WalkerLane L;
+    L.lane = threadIdx.x & 63;
+    L.wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int Ne = t.nelec;
+    L.G = 64 / Ne;
+    L.wl = L.lane / Ne;
+    L.lane_real = L.wl < L.G;
+    L.wbase = L.lane_real ? L.wl * Ne : 64 - Ne;
+    L.e = L.lane - L.wbase;
+    const int64_t w = (int64_t)L.wave * L.G + (L.lane_real ? L.wl : 0);
+    L.valid = L.lane_real && w < nw;
+    L.w = (int)w;
+    L.wclamp = (int)(w < nw ? w : nw - 1);
+    L.spin = L.e < t.nup ? 0 : 1;
+    L.myrow = L.spin == 0 ? L.e : L.e - t.nup;
+    L.gbase = L.spin == 0 ? L.wbase : L.wbase + t.nup;
+    L.n_sig = L.spin == 0 ? t.nup : t.ndown;
+    L.gmask = (L.n_sig >= 64) ? ~0ull : (((1ull << L.n_sig) - 1ull) << L.gbase);
+    return L;
+}
+
+// MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95)
+template <typename T, int NM, bool DERIV>
+struct MoSink {
+    const T* __restrict__ W;
+    int nmo;
+    T* mo;
+    T* gx;
+    T* gy;
+    T* gz;
+    T* lp;
+    __device__ __forceinline__ void operator()(int mu, T ao, T ax, T ay, T az, T al) const {
+        const T* __restrict__ wr = W + (size_t)mu * nmo;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {
+            if (m < nmo) {
+                const T w = wr[m];
+                mo[m] += ao * w;
+                if (DERIV) {
+                    gx[m] += ax * w;
+                    gy[m] += ay * w;
+                    gz[m] += az * w;
+                    lp[m] += al * w;
+                }
+            }
+        }
+    }
+};
+
+// Pade Jastrow terms of one electron (jastrow_factor_electron_electron.py:124-260,
+// jastrow_factor_electron_nuclei.py:56-161, pade kernels): value contributions,
+// gradient of ln J with respect to this electron and its Laplacian of ln J;
+// also the potential terms of wf_base.py:49-95 (V_ee counted half per lane).
+template <bool DERIV>
+__device__ __forceinline__ void jastrow_and_potential(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                                      double x, double y, double z, double& kee, double& ken,
+                                                      double& gx, double& gy, double& gz, double& lap,
+                                                      double& vpot) {
+    kee = ken = gx = gy = gz = lap = vpot = 0.0;
+    const int Ne = t.nelec;
+    const double wee = (t.use_ee && prm.w_ee) ? prm.w_ee[0] : 0.0;
+    const double wen = (t.use_en && prm.w_en) ? prm.w_en[0] : 0.0;
+    for (int j = 0; j < Ne; ++j) {
+        const double xj = __shfl(x, L.wbase + j);
+        const double yj = __shfl(y, L.wbase + j);
+        const double zj = __shfl(z, L.wbase + j);
+        if (j != L.e) {
+            const double dx = x - xj, dy = y - yj, dz = z - zj;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ir = 1.0 / r;
+            if (DERIV) vpot += 0.5 * ir;
+            if (t.use_ee) {
+                const bool same = (j < t.nup) == (L.e < t.nup);
+                const double w0 = same ? 0.25 : 0.5;
+                const double den = 1.0 / (1.0 + wee * r);
+                kee += w0 * r * den;
+                if (DERIV) {
+                    const double kp = w0 * den * den;
+                    const double kpp = -2.0 * w0 * wee * den * den * den;
+                    gx += kp * dx * ir;
+                    gy += kp * dy * ir;
+                    gz += kp * dz * ir;
+                    lap += kpp + 2.0 * kp * ir;
+                }
+            }
+        }
+    }
+    if (t.use_en) {
+        for (int A = 0; A < t.natom; ++A) {
+            const double dx = x - t.jas_atoms[3 * A], dy = y - t.jas_atoms[3 * A + 1], dz = z - t.jas_atoms[3 * A + 2];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double den = 1.0 / (1.0 + wen * r);
+            ken += r * den;
+            if (DERIV) {
+                const double ir = 1.0 / r;
+                const double kp = den * den;
+                const double kpp = -2.0 * wen * den * den * den;
+                gx += kp * dx * ir;
+                gy += kp * dy * ir;
+                gz += kp * dz * ir;
+                lap += kpp + 2.0 * kp * ir;
+            }
+        }
+    }
+    if (DERIV) {
+        for (int A = 0; A < t.natom; ++A) {
+            const double dx = x - prm.atom_coords[3 * A], dy = y - prm.atom_coords[3 * A + 1],
+                         dz = z - prm.atom_coords[3 * A + 2];
+            vpot -= t.atom_z[A] / sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    }
+}
+
+// Gather row entries of configuration c from an MO-like row (uniform c).
+template <int NM, typename S>
+__device__ __forceinline__ void gather_conf(const Tables& t, const WalkerLane& L, int c, const S (&src)[NM],
+                                            double (&dst)[NM]) {
+    if (t.identity_conf) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) dst[j] = (double)src[j];
+        return;
+    }
+    const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+    const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
+    const bool has = c < nconf;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        double v = 0.0;
+        if (has && j < L.n_sig) {
+            const int idx = conf[j];
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m == idx) v = (double)src[m];
+        }
+        dst[j] = v;
+    }
+}
+
+// Slater part for one lane: determinants (and traces) of every unique
+// configuration of the lane's spin, then the CI combination at the walker's
+// first lane (slater_pooling.py:64-111, 262-409; slater_jastrow.py:312-344).
+// Returns log|sum_n c_n D_n| and its sign in every lane of the walker, and the
+// Jacobi kinetic ratio sum_n c_n (tr_up + tr_dn) D_n / sum_n c_n D_n.
+template <int NM, bool KIN, typename T>
+__device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                               const T (&mo)[NM], const double (&bk)[NM], double& logabs,
+                                               double& sgn, double& kin) {
+    double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        ld[c] = 0.0;
+        sd[c] = 1.0;
+        td[c] = 0.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < t.nconf_max; ++c) {
+        double a[NM], b[NM];
+        gather_conf<NM>(t, L, c, mo, a);
+        if (KIN) gather_conf<NM>(t, L, c, bk, b);
+        double lg, sg, tp;
+        int col;
+        lane_gauss_jordan<NM, KIN>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        const double tr = KIN ? seg_sum(tp, L.gbase, L.n_sig, t.nsig_max) : 0.0;
+#pragma unroll
+        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2) {
+            if (c2 == c) {
+                ld[c2] = lg;
+                sd[c2] = sg;
+                td[c2] = tr;
+            }
+        }
+    }
+    // exchange: every lane fetches the up values (lane wbase) and the down
+    // values (lane wbase + nup) of its walker
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    double lu[QMC_MAXCONF], su[QMC_MAXCONF], tu[QMC_MAXCONF];
+    double lo[QMC_MAXCONF], so[QMC_MAXCONF], to[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        if (c < t.nconf_max) {
+            lu[c] = __shfl(ld[c], up_lane);
+            su[c] = __shfl(sd[c], up_lane);
+            lo[c] = __shfl(ld[c], dn_lane);
+            so[c] = __shfl(sd[c], dn_lane);
+            if (KIN) {
+                tu[c] = __shfl(td[c], up_lane);
+                to[c] = __shfl(td[c], dn_lane);
+            }
+        }
+        if (t.ndown == 0) {
+            lo[c] = 0.0;
+            so[c] = 1.0;
+            to[c] = 0.0;
+        }
+    }
+    double lmax = -INFINITY;
+    for (int n = 0; n < t.nci; ++n) {
+        const int cu = t.ci_up[n], cd = t.ci_dn[n];
+        double l = 0.0;
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            if (c == cu) l += lu[c];
+            if (c == cd) l += lo[c];
+        }
+        lmax = fmax(lmax, l);
+    }
+    if (!(lmax > -INFINITY)) lmax = 0.0;
+    double s = 0.0, st = 0.0;
+    for (int n = 0; n < t.nci; ++n) {
+        const int cu = t.ci_up[n], cd = t.ci_dn[n];
+        double l = 0.0, sg = 1.0, tr = 0.0;
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            if (c == cu) {
+                l += lu[c];
+                sg *= su[c];
+                if (KIN) tr += tu[c];
+            }
+            if (c == cd) {
+                l += lo[c];
+                sg *= so[c];
+                if (KIN) tr += to[c];
+            }
+        }
+        const double dn = prm.ci[n] * sg * exp(l - lmax);
+        s += dn;
+        if (KIN) st += tr * dn;
+    }
+    logabs = lmax + log(fabs(s));
+    sgn = s < 0.0 ? -1.0 : 1.0;
+    kin = KIN ? st / s : 0.0;
+}
+
+struct KArgs {
+    Tables t;
+    QmcParams p;
+    const double* pos;
+    int64_t nw;
+    QmcWfOut out;
+};
+
+// psi / kinetic / local energy of nw walkers.
+template <typename T, int NM, int RAD, bool ELOC>
+__global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
+    const Tables& t = a.t;
+    const WalkerLane L = lane_layout(t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;  // whole wavefront beyond the batch
+    const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
+    const double x = p[0], y = p[1], z = p[2];
+
+    T mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = T(0);
+    MoSink<T, NM, ELOC> sink{(const T*)a.p.mo_w, t.nmo, mo, gx, gy, gz, lp};
+    ao_row<T, RAD, ELOC>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+
+    double kee, ken, jgx, jgy, jgz, jlap, vpot;
+    jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vpot);
+
+    double bk[NM];
+    if (ELOC) {
+        const double l2 = jlap + jgx * jgx + jgy * jgy + jgz * jgz;  // (lap J)/J
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+            bk[m] = -0.5 * ((double)lp[m] + 2.0 * (jgx * (double)gx[m] + jgy * (double)gy[m] + jgz * (double)gz[m]) +
+                            l2 * (double)mo[m]);
+    }
+    double logabs, sgn, kin;
+    slater_combine<NM, ELOC>(t, a.p, L, mo, bk, logabs, sgn, kin);
+    const double logj = seg_sum(0.5 * kee + ken, L.wbase, t.nelec, t.nelec);
+    double epot = 0.0;
+    if (ELOC) epot = seg_sum(vpot, L.wbase, t.nelec, t.nelec) + t.vnn;
+    if (L.valid && L.e == 0) {
+        const double la = logabs + logj;
+        if (a.out.logabs) a.out.logabs[L.w] = la;
+        if (a.out.sign) a.out.sign[L.w] = sgn;
+        if (a.out.psi) a.out.psi[L.w] = sgn * exp(la);
+        if (ELOC) {
+            if (a.out.kinetic) a.out.kinetic[L.w] = kin;
+            if (a.out.epot) a.out.epot[L.w] = epot;
+            if (a.out.eloc) a.out.eloc[L.w] = kin + epot;
+        }
+    }
+}
+
+struct SArgs {
+    Tables t;
+    QmcParams p;
+    QmcWalkers wk;
+    QmcSweepDesc sw;
+    int64_t nw;
+};
+
+template <typename T, int NM, int RAD>
+__device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                              double y, double z, double& sgn) {
+    T mo[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = T(0);
+    MoSink<T, NM, false> sink{(const T*)prm.mo_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
+    ao_row<T, RAD, false>(t, prm.bas_exp, prm.atom_coords, x, y, z, sink);
+    double kee, ken, d0, d1, d2, d3, d4;
+    jastrow_and_potential<false>(t, prm, L, x, y, z, kee, ken, d0, d1, d2, d3, d4);
+    double bk[NM];
+    double logabs, kin;
+    slater_combine<NM, false>(t, prm, L, mo, bk, logabs, sgn, kin);
+    return logabs + seg_sum(0.5 * kee + ken, L.wbase, t.nelec, t.nelec);
+}
+
+// Persistent Metropolis sweep: nstep all-electron moves per launch, walker
+// state in registers (sampler/metropolis.py:134-165, 256-298).
+template <typename T, int NM, int RAD>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
+    const Tables& t = a.t;
+    const WalkerLane L = lane_layout(t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    const int Ne = t.nelec;
+    double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
+    double x = pp[0], y = pp[1], z = pp[2];
+    double sgn;
+    double lp;
+    if (a.sw.init_logpsi) {
+        lp = eval_logpsi<T, NM, RAD>(t, a.p, L, x, y, z, sgn);
+    } else {
+        lp = a.wk.logabs[L.wclamp];
+        sgn = a.wk.sign[L.wclamp];
+    }
+    // psi == 0 is treated as psi^2 = eps (metropolis.py:103-107,124)
+    const double lp_floor = -18.420680743952367;  // 0.5 * log(1e-16)
+    if (!(lp > lp_floor)) lp = lp_floor;
+    const uint64_t wid = a.wk.walker_id ? (uint64_t)a.wk.walker_id[L.wclamp] : (uint64_t)L.wclamp;
+    const uint32_t k0 = (uint32_t)a.sw.seed, k1 = (uint32_t)(a.sw.seed >> 32);
+    const double sd = sqrt(a.sw.sigma2);
+    int nacc = 0;
+    for (int s = 0; s < a.sw.nstep; ++s) {
+        const uint64_t step = (uint64_t)(a.sw.step0 + s);
+        // counter: (walker id lo, walker id hi, step, electron | stream<<16)
+        const u4 ctr{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, (uint32_t)L.e};
+        const u4 rnd = philox4x32_10(ctr, k0, k1);
+        double nx, ny, nz;
+        if (a.sw.uniform_move) {
+            nx = a.sw.step_size * (2.0 * u01_closed0(rnd.x) - 1.0);
+            ny = a.sw.step_size * (2.0 * u01_closed0(rnd.y) - 1.0);
+            nz = a.sw.step_size * (2.0 * u01_closed0(rnd.z) - 1.0);
+        } else {
+            // Box-Muller on (rnd.x, rnd.y) and (rnd.z, rnd.w)
+            const double r1 = sqrt(-2.0 * log(u01_open0(rnd.x)));
+            const double r2 = sqrt(-2.0 * log(u01_open0(rnd.z)));
+            double s1, c1, s2, c2;
+            sincospi(2.0 * u01_closed0(rnd.y), &s1, &c1);
+            sincospi(2.0 * u01_closed0(rnd.w), &s2, &c2);
+            nx = sd * r1 * c1;
+            ny = sd * r1 * s1;
+            nz = sd * r2 * c2;
+        }
+        const double xn = x + nx, yn = y + ny, zn = z + nz;
+        double sgn_n;
+        double lpn = eval_logpsi<T, NM, RAD>(t, a.p, L, xn, yn, zn, sgn_n);
+        if (!(lpn > lp_floor)) lpn = lp_floor;
+        // acceptance uniform: electron slot nelec of the same counter family
+        const u4 ua = philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u}, k0, k1);
+        const double U = u01_closed0(ua.x);
+        const double ratio = exp(fmin(2.0 * (lpn - lp), 0.0));
+        const bool acc = ratio - U >= 0.0;
+        if (acc) {
+            x = xn;
+            y = yn;
+            z = zn;
+            lp = lpn;
+            sgn = sgn_n;
+            nacc++;
+        }
+        if (a.sw.keep && L.valid && (int64_t)step >= a.sw.keep_from &&
+            ((int64_t)step - a.sw.keep_from) % a.sw.keep_every == 0) {
+            const int64_t ik = ((int64_t)step - a.sw.keep_from) / a.sw.keep_every;
+            double* kp = a.sw.keep + ((size_t)ik * a.nw + L.w) * 3 * Ne + 3 * L.e;
+            kp[0] = x;
+            kp[1] = y;
+            kp[2] = z;
+        }
+    }
+    if (L.valid) {
+        pp[0] = x;
+        pp[1] = y;
+        pp[2] = z;
+        if (L.e == 0) {
+            a.wk.logabs[L.w] = lp;
+            a.wk.sign[L.w] = sgn;
+            if (a.wk.naccept) a.wk.naccept[L.w] += nacc;
+        }
+    }
+}
+
+// AO values and derivatives in the reference layout (atomic_orbitals.py:578-609).
+struct AArgs {
+    Tables t;
+    QmcParams p;
+    const double* pos;
+    int64_t nw;
+    double* ao;
+    double* dao;
+    double* d2ao;
+};
+
+template <int RAD, bool DERIV>
+__global__ __launch_bounds__(kBlock) void ao_kernel(AArgs a) {
+    const Tables& t = a.t;
+    const WalkerLane L = lane_layout(t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
+    const size_t row = ((size_t)L.wclamp * t.nelec + L.e) * t.nao;
+    const bool v = L.valid;
+    double* ao = a.ao;
+    double* dao = a.dao;
+    double* d2ao = a.d2ao;
+    auto sink = [&](int mu, double val, double ax, double ay, double az, double al) {
+        if (!v) return;
+        ao[row + mu] += val;
+        if (DERIV) {
+            if (dao) {
+                dao[(row + mu) * 3 + 0] += ax;
+                dao[(row + mu) * 3 + 1] += ay;
+                dao[(row + mu) * 3 + 2] += az;
+            }
+            if (d2ao) d2ao[row + mu] += al;
+        }
+    };
+    ao_row<double, RAD, DERIV>(t, a.p.bas_exp, a.p.atom_coords, p[0], p[1], p[2], sink);
+}
+
+// Adjoint rows of log|psi| (see qmc_logpsi_grad_rows in the header).
+struct GArgs {
+    Tables t;
+    QmcParams p;
+    const double* pos;
+    int64_t nw;
+    double* ao_rows;     // [nw*nelec][nao]
+    double* gamma_rows;  // [nw*nelec][nmo]
+    double* dci;         // [nw][nci]
+    double* djas;        // [nw][2]
+    double* dexp_rows;   // [nw*nelec][nprim_ref] or null
+};
+
+// log|det| and sign of every unique configuration of the lane's spin; with
+// INV the inverse column of this lane's row, weighted by om[c] and scattered
+// to MO columns, is accumulated into gam (second pass of the adjoint).
+template <int NM, bool INV>
+__device__ __forceinline__ void conf_dets(const Tables& t, const WalkerLane& L, const double (&mo)[NM],
+                                          double (&ld)[QMC_MAXCONF], double (&sd)[QMC_MAXCONF],
+                                          const double (&om)[QMC_MAXCONF], double (&gam)[NM]) {
+#pragma unroll 1
+    for (int c = 0; c < t.nconf_max; ++c) {
+        double aa[NM], bb[NM];
+        gather_conf<NM>(t, L, c, mo, aa);
+#pragma unroll
+        for (int j = 0; j < NM; ++j) bb[j] = (j == L.myrow) ? 1.0 : 0.0;
+        double lg, sg, tp;
+        int col;
+        lane_gauss_jordan<NM, INV>(aa, bb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+#pragma unroll
+        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+            if (c2 == c) {
+                ld[c2] = lg;
+                sd[c2] = sg;
+            }
+        if (INV) {
+            double w = 0.0;
+#pragma unroll
+            for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+                if (c2 == c) w = om[c2];
+            const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+            const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
+            // the lane that pivoted column k holds row k of A^-1 in bb; this
+            // lane needs (A^-1)[k][myrow] for every k
+            for (int k = 0; k < t.nsig_max; ++k) {
+                int src = 0;
+                for (int s = 0; s < t.nsig_max; ++s) {
+                    const int cs = __shfl(col, L.gbase + (s < L.n_sig ? s : 0));
+                    if (s < L.n_sig && cs == k) src = s;
+                }
+                double inv_ki = 0.0;
+#pragma unroll
+                for (int j = 0; j < NM; ++j) {
+                    const double vj = __shfl(bb[j], L.gbase + src);
+                    if (j == L.myrow) inv_ki = vj;
+                }
+                if (k < L.n_sig && c < nconf) {
+                    const int mcol = t.identity_conf ? k : conf[k];
+#pragma unroll
+                    for (int m = 0; m < NM; ++m)
+                        if (m == mcol) gam[m] += w * inv_ki;
+                }
+            }
+        }
+    }
+}
+
+// d log|psi| / d bas_exp for one electron: sum over the AO terms of
+// G[mu] * d AO_mu / d a_p with G[mu] = sum_m Gamma[m] W[mu][m] and
+// d/da of K r^n e^{-a r^2} Y  = -r^2 (term)   (GTO)
+// d/da of K r^n e^{-a r} Y    = -r   (term)   (STO)
+template <int NM, int RAD>
+__device__ __forceinline__ void exp_adjoint(const Tables& t, const QmcParams& prm, double ex, double ey, double ez,
+                                            const double (&gam)[NM], double* out) {
+    const double* W = (const double*)prm.mo_w;
+    for (int g = 0; g < t.ngroup; ++g) {
+        const int at = t.grp_atom[g];
+        const double dx = ex - prm.atom_coords[3 * at], dy = ey - prm.atom_coords[3 * at + 1],
+                     dz = ez - prm.atom_coords[3 * at + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
+        const int np = t.grp_nprim[g], p0 = t.grp_prim0[g];
+        const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
+        for (int i = i0; i < i1; ++i) {
+            const int mu = t.gao_mu[i];
+            double G = 0.0;
+            const double* wr = W + (size_t)mu * t.nmo;
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m < t.nmo) G += gam[m] * wr[m];
+            const int kk = t.gao_k[i];
+            double Y, d0, d1, d2, d3;
+            cart_poly<double, false>(dx, dy, dz, kk & 15, (kk >> 4) & 15, (kk >> 8) & 15, Y, d0, d1, d2, d3);
+            for (int p = 0; p < np; ++p) {
+                const double a = prm.bas_exp[t.grp_prim_src[p0 + p]];
+                const double K = t.gao_K[(size_t)i * QMC_MAXP + p];
+                double term;
+                if (RAD == QMC_RADIAL_GTO_PURE) {
+                    term = -r2 * K * exp(-a * r2) * Y;
+                } else {
+                    const int n = (int)t.gao_n[(size_t)i * QMC_MAXP + p];
+                    term = -r * K * ipow(r, n) * exp(-a * r) * Y;
+                }
+                if (out) out[t.gao_src[(size_t)i * QMC_MAXP + p]] = G * term;
+            }
+        }
+    }
+}
+
+// Adjoint rows of log|psi| (qmc_logpsi_grad_rows): AO row, Gamma row
+// (d log|psi| / d MO[i, m]), D_n/S per CI term and the Jastrow-weight derivatives.
+template <int NM, int RAD>
+__global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
+    const Tables& t = a.t;
+    const WalkerLane L = lane_layout(t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
+    const double x = p[0], y = p[1], z = p[2];
+    const size_t row = (size_t)L.wclamp * t.nelec + L.e;
+    const bool v = L.valid;
+    double mo[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = 0.0;
+    {
+        double* aor = a.ao_rows + row * t.nao;
+        const double* W = (const double*)a.p.mo_w;
+        auto sink = [&](int mu, double val, double, double, double, double) {
+            if (v) aor[mu] += val;
+            const double* wr = W + (size_t)mu * t.nmo;
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m < t.nmo) mo[m] += val * wr[m];
+        };
+        ao_row<double, RAD, false>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+    }
+    double ld[QMC_MAXCONF], sd[QMC_MAXCONF], om[QMC_MAXCONF], gam[NM];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        ld[c] = 0.0;
+        sd[c] = 1.0;
+        om[c] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < NM; ++m) gam[m] = 0.0;
+    conf_dets<NM, false>(t, L, mo, ld, sd, om, gam);
+    // CI weights omega_n = c_n D_n / S with D_n = D_up D_dn (slater_pooling.py:96-111)
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    double lu[QMC_MAXCONF], su[QMC_MAXCONF], lo[QMC_MAXCONF], so[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        lu[c] = __shfl(ld[c], up_lane);
+        su[c] = __shfl(sd[c], up_lane);
+        const double l2 = __shfl(ld[c], dn_lane), s2 = __shfl(sd[c], dn_lane);
+        lo[c] = t.ndown > 0 ? l2 : 0.0;
+        so[c] = t.ndown > 0 ? s2 : 1.0;
+    }
+    auto term = [&](int n, double& l, double& sg) {
+        l = 0.0;
+        sg = 1.0;
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            if (c == t.ci_up[n]) {
+                l += lu[c];
+                sg *= su[c];
+            }
+            if (c == t.ci_dn[n]) {
+                l += lo[c];
+                sg *= so[c];
+            }
+        }
+    };
+    double lmax = -INFINITY;
+    for (int n = 0; n < t.nci; ++n) {
+        double l, sg;
+        term(n, l, sg);
+        lmax = fmax(lmax, l);
+    }
+    if (!(lmax > -INFINITY)) lmax = 0.0;
+    double S = 0.0;
+    for (int n = 0; n < t.nci; ++n) {
+        double l, sg;
+        term(n, l, sg);
+        S += a.p.ci[n] * sg * exp(l - lmax);
+    }
+    for (int n = 0; n < t.nci; ++n) {
+        double l, sg;
+        term(n, l, sg);
+        const double dn_s = sg * exp(l - lmax) / S;  // D_n / S = d log|psi| / d c_n
+        const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c)
+            if (c == mine) om[c] += a.p.ci[n] * dn_s;
+        if (v && L.e == 0) a.dci[(size_t)L.w * t.nci + n] = dn_s;
+    }
+    conf_dets<NM, true>(t, L, mo, ld, sd, om, gam);
+    if (v) {
+        double* gr = a.gamma_rows + row * t.nmo;
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+            if (m < t.nmo) gr[m] = gam[m];
+    }
+    if (a.dexp_rows) exp_adjoint<NM, RAD>(t, a.p, x, y, z, gam, v ? a.dexp_rows + row * t.nprim_ref : nullptr);
+    // Jastrow weights: d/dw [w0 r/(1+w r)] = -w0 r^2/(1+w r)^2
+    double dee = 0.0, den_ = 0.0;
+    const double wee = (t.use_ee && a.p.w_ee) ? a.p.w_ee[0] : 0.0;
+    const double wen = (t.use_en && a.p.w_en) ? a.p.w_en[0] : 0.0;
+    for (int j = 0; j < t.nelec; ++j) {
+        const double xj = __shfl(x, L.wbase + j), yj = __shfl(y, L.wbase + j), zj = __shfl(z, L.wbase + j);
+        if (t.use_ee && j != L.e) {
+            const double r = sqrt((x - xj) * (x - xj) + (y - yj) * (y - yj) + (z - zj) * (z - zj));
+            const double w0 = ((j < t.nup) == (L.e < t.nup)) ? 0.25 : 0.5;
+            const double q = 1.0 / (1.0 + wee * r);
+            dee -= 0.5 * w0 * r * r * q * q;
+        }
+    }
+    if (t.use_en) {
+        for (int A = 0; A < t.natom; ++A) {
+            const double dx = x - t.jas_atoms[3 * A], dy = y - t.jas_atoms[3 * A + 1], dz = z - t.jas_atoms[3 * A + 2];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double q = 1.0 / (1.0 + wen * sqrt(r2));
+            den_ -= r2 * q * q;
+        }
+    }
+    const double sdee = seg_sum(dee, L.wbase, t.nelec, t.nelec);
+    const double sden = seg_sum(den_, L.wbase, t.nelec, t.nelec);
+    if (v && L.e == 0) {
+        a.djas[(size_t)L.w * 2 + 0] = sdee;
+        a.djas[(size_t)L.w * 2 + 1] = sden;
+    }
+}
+
+// Zero fill, a kernel rather than a memset node so that the entry points stay
+// safe to record in HIP graphs.
+static __global__ __launch_bounds__(kBlock) void fill_zero_kernel(double* p, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += stride) p[k] = 0.0;
+}
+
+inline void fill_zero(double* p, size_t n, hipStream_t s) {
+    if (!p || n == 0) return;
+    size_t blocks = (n + kBlock - 1) / kBlock;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, p, n);
+}
+
+}  // namespace
+

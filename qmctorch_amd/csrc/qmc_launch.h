@@ -1,0 +1,33 @@
+// Host launchers of the kernel families, one translation unit each so the
+// library builds in parallel.  Every launcher returns 0, or -1 when the
+// orbital count has no compiled bucket.
+#pragma once
+#include "qmc_kernels.h"
+
+namespace qmc {
+
+// register-array bucket for max(nmo, nup)
+inline int nm_bucket(const Tables& t) {
+    const int need = t.nmo > t.nup ? t.nmo : t.nup;
+    if (need <= 2) return 2;
+    if (need <= 4) return 4;
+    if (need <= 8) return 8;
+    if (need <= 16) return 16;
+    if (need <= 24) return 24;
+    return -1;
+}
+
+inline dim3 grid_for(const Tables& t, int64_t nw) {
+    const int G = 64 / t.nelec;
+    const int64_t waves = (nw + G - 1) / G;
+    return dim3((unsigned)((waves + (kBlock / 64) - 1) / (kBlock / 64)));
+}
+
+int launch_wf_f32(const KArgs& a, int mode, hipStream_t s);
+int launch_wf_f64(const KArgs& a, int mode, hipStream_t s);
+int launch_sweep_f32(const SArgs& a, hipStream_t s);
+int launch_sweep_f64(const SArgs& a, hipStream_t s);
+int launch_grad(const GArgs& a, hipStream_t s);
+int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
+
+}  // namespace qmc

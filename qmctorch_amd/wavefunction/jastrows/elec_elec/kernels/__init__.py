@@ -1,0 +1,3 @@
+from .pade_jastrow_kernel import PadeJastrowKernel
+
+__all__ = ["PadeJastrowKernel"]

@@ -1,0 +1,320 @@
+"""Slater-Jastrow wavefunction on MI355X (wavefunction/slater_jastrow.py:30-482).
+
+Same constructor, attributes, sub-modules and state_dict keys as the reference
+(``ao.atom_coords``, ``ao.bas_exp``, ``mo.mo_modifier``, ``fc.weight``,
+``jastrow.jastrow_kernel.weight`` / ``jastrow.jastrow_terms.<i>...``).  The
+numerical work -- AO -> MO -> Slater determinants -> CI sum -> Jastrow, the
+Jacobi kinetic energy and the potentials -- runs in the fused kernels of
+libqmcmi355x (``qmc_wf_eval``); parameter gradients of ``forward`` come from
+the adjoint kernel ``qmc_logpsi_grad_rows`` instead of autograd.
+
+Extra keyword ``precision``: ``"fp64"`` (default, the reference's double
+precision end to end) or ``"mixed"`` (AO, MO and Jastrow in FP32; Slater
+elimination, traces and energies in FP64).
+"""
+from typing import Optional, Union
+
+import numpy as np
+import torch
+from torch import nn
+
+from .. import _native as N
+from .. import log
+from .jastrows.combine_jastrow import CombineJastrow
+from .jastrows.elec_elec.jastrow_factor_electron_electron import JastrowFactorElectronElectron
+from .jastrows.elec_elec.kernels.pade_jastrow_kernel import PadeJastrowKernel
+from .orbitals.atomic_orbitals import AtomicOrbitals
+from .orbitals.molecular_orbitals import MolecularOrbitals
+from .plan import DevicePlan, SystemTables
+from .pooling.orbital_configurations import OrbitalConfigurations
+from .pooling.slater_pooling import SlaterPooling
+from .wf_base import WaveFunction
+
+_PREC = {"fp64": N.QMC_PREC_FP64, "mixed": N.QMC_PREC_MIXED}
+
+
+class SlaterJastrow(WaveFunction):
+    def __init__(
+        self,
+        mol,
+        jastrow: Optional[Union[str, nn.Module, list]] = "default",
+        backflow=None,
+        configs: str = "ground_state",
+        kinetic: str = "jacobi",
+        cuda: bool = False,
+        include_all_mo: bool = True,
+        mix_mo: bool = False,
+        orthogonalize_mo: bool = False,
+        precision: str = "fp64",
+    ) -> None:
+        super().__init__(mol.nelec, 3, kinetic, cuda)
+        if backflow is not None:
+            raise NotImplementedError("backflow orbitals are not part of the MI355X path")
+        if kinetic != "jacobi":
+            raise NotImplementedError("kinetic='%s': the device path computes the kinetic energy by the Jacobi formula"
+                                      % kinetic)
+        if not include_all_mo and isinstance(configs, str) and configs.startswith("cas("):
+            raise ValueError("CAS calculation only possible with include_all_mo=True")
+        if precision not in _PREC:
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.mol = mol
+        self.atoms = mol.atoms
+        self.natom = mol.natom
+
+        # configurations (slater_jastrow.py:158-169)
+        self.orb_confs = OrbitalConfigurations(mol)
+        self.configs_method = configs if isinstance(configs, str) else "explicit"
+        self.configs = self.orb_confs.get_configs(configs)
+        self.nci = len(self.configs[0])
+        mx = [int(c.max()) for c in self.configs if c.numel()]
+        self.highest_occ_mo = max(mx) + 1
+
+        self.use_backflow = False
+        self.ao = AtomicOrbitals(mol, cuda)
+        self.include_all_mo = include_all_mo
+        self.nmo_opt = int(mol.basis.nmo) if include_all_mo else self.highest_occ_mo
+        self.mo = MolecularOrbitals(mol, include_all_mo, self.highest_occ_mo, mix_mo, orthogonalize_mo, cuda)
+        self.pool = SlaterPooling(self.configs_method, self.configs, mol, cuda)
+
+        # CI layer, ground state weight 1 (slater_jastrow.py:179-191)
+        self.fc = nn.Linear(self.nci, 1, bias=False)
+        self.fc.weight.data.fill_(0.0)
+        self.fc.weight.data[0][0] = 1.0
+        if cuda:
+            self.fc = self.fc.to(self.device)
+
+        # Jastrow (slater_jastrow.py:193-222)
+        if jastrow is None:
+            self.jastrow = None
+            self.use_jastrow = False
+        else:
+            self.use_jastrow = True
+            if isinstance(jastrow, str) and jastrow == "default":
+                self.jastrow = JastrowFactorElectronElectron(mol, PadeJastrowKernel, cuda=cuda)
+            elif isinstance(jastrow, list):
+                self.jastrow = CombineJastrow(jastrow)
+            elif isinstance(jastrow, nn.Module):
+                self.jastrow = jastrow
+            else:
+                raise TypeError("Jastrow factor not supported.")
+            self.jastrow_type = self.jastrow.__repr__()
+            if cuda:
+                self.jastrow = self.jastrow.to(self.device)
+
+        self.kinetic_method = kinetic
+        self.kinetic_energy = self.kinetic_energy_jacobi
+        self.gradients = self.gradients_jacobi
+        self._plan = None
+        self._plan_exp_version = None
+        self._plan_tables = None
+        self.log_data()
+
+    # ------------------------------------------------------------------ plan
+    def _jastrow_terms(self):
+        if not self.use_jastrow:
+            return None, None
+        terms = self.jastrow.terms() if isinstance(self.jastrow, CombineJastrow) else [self.jastrow]
+        ee = en = None
+        for t in terms:
+            kind = getattr(t, "kind", None)
+            if kind == "ee":
+                ee = t
+            elif kind == "en":
+                en = t
+            else:
+                raise NotImplementedError("Jastrow term %r is not supported on the device" % t)
+        return ee, en
+
+    def device_plan(self) -> DevicePlan:
+        """Plan for the current exponents (rebuilt if exponent sharing breaks)."""
+        ver = self.ao.bas_exp._version
+        if self._plan is not None and self._plan_exp_version == ver:
+            return self._plan
+        ex = self.ao.bas_exp.detach().to("cpu", torch.float64).numpy()
+        if self._plan is not None and self._plan_tables.grouping_valid(ex):
+            self._plan_exp_version = ver
+            return self._plan
+        ee, en = self._jastrow_terms()
+        tab = SystemTables(
+            self.mol.basis, self.mol.atom_coords, self.mol.atomic_number, self.mol.nup, self.mol.ndown,
+            (self.configs[0].numpy(), self.configs[1].numpy()), self.highest_occ_mo,
+            ee is not None, en is not None, self.ao.norm_cst.cpu().numpy(), ex,
+        )
+        self._plan = DevicePlan(tab)
+        self._plan_tables = tab
+        self._plan_exp_version = ver
+        return self._plan
+
+    def _params(self, precision: str):
+        """Parameter views handed to the kernels (all tiny device tensors)."""
+        ee, en = self._jastrow_terms()
+        f64 = torch.float64
+        W = self.mo.weight()[:, : self.highest_occ_mo]
+        W = W.detach().to(torch.float32 if precision == "mixed" else f64).contiguous()
+        keep = {
+            "bas_exp": self.ao.bas_exp.detach().to(f64).contiguous(),
+            "atom_coords": self.ao.atom_coords.detach().to(f64).contiguous(),
+            "mo_w": W,
+            "ci": self.fc.weight.detach().reshape(-1).to(f64).contiguous(),
+            "w_ee": ee.jastrow_kernel.weight.detach().to(f64).contiguous() if ee is not None else None,
+            "w_en": en.jastrow_kernel.weight.detach().to(f64).contiguous() if en is not None else None,
+        }
+        p = N.QmcParams()
+        for k, v in keep.items():
+            setattr(p, k, N.ptr(v))
+        return p, keep
+
+    def _check_pos(self, pos: torch.Tensor) -> torch.Tensor:
+        N.require_device(pos, "positions")
+        if pos.dim() != 2 or pos.shape[1] != self.nelec * 3:
+            raise ValueError("positions must have shape (Nbatch, %d)" % (self.nelec * 3))
+        return pos.detach().to(torch.float64).contiguous()
+
+    def _eval(self, pos: torch.Tensor, mode: int, precision: Optional[str] = None):
+        precision = precision or self.precision
+        x = self._check_pos(pos)
+        nw = x.shape[0]
+        plan = self.device_plan()
+        prm, keep = self._params(precision)
+        out = {k: torch.empty(nw, dtype=torch.float64, device=x.device)
+               for k in (("psi", "logabs", "sign") if mode == N.QMC_EVAL_PSI
+                         else ("psi", "logabs", "sign", "kinetic", "eloc", "epot"))}
+        o = N.QmcWfOut()
+        for k, v in out.items():
+            setattr(o, k, v.data_ptr())
+        N.check(N.load().qmc_wf_eval(plan.handle, prm, _PREC[precision], mode, x.data_ptr(), nw, o,
+                                     N.current_stream()))
+        return out
+
+    # ---------------------------------------------------------- public API
+    def forward(self, x: torch.Tensor, ao: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """psi = J * sum_n c_n D_up D_dn, shape (Nbatch, 1) (slater_jastrow.py:243-286).
+
+        Differentiable with respect to ``mo.mo_modifier``, ``fc.weight``,
+        the Jastrow weights and ``ao.bas_exp`` through the adjoint kernel."""
+        if ao is not None:
+            raise NotImplementedError("forward(x, ao=...) is not supported by the fused device path")
+        ee, en = self._jastrow_terms()
+        params = [self.mo.mo_modifier, self.fc.weight, self.ao.bas_exp,
+                  ee.jastrow_kernel.weight if ee is not None else None,
+                  en.jastrow_kernel.weight if en is not None else None]
+        if self.ao.atom_coords.requires_grad and torch.is_grad_enabled() and not getattr(self, "_warned_geo", False):
+            log.warning("ao.atom_coords requires grad: the device adjoint does not produce gradients with "
+                        "respect to atomic positions (forces are not part of this path)")
+            self._warned_geo = True
+        return _PsiFunction.apply(self, x, *params)
+
+    def pdf(self, pos: torch.Tensor, return_grad: Optional[bool] = False) -> torch.Tensor:
+        if return_grad:
+            return self.gradients(pos, pdf=True)
+        psi = self._eval(pos, N.QMC_EVAL_PSI)["psi"]
+        return (psi * psi).to(torch.get_default_dtype())
+
+    def log_psi(self, pos: torch.Tensor, precision: Optional[str] = None):
+        """(log|psi|, sign(psi)) in float64, shape (Nbatch,)."""
+        out = self._eval(pos, N.QMC_EVAL_PSI, precision)
+        return out["logabs"], out["sign"]
+
+    def kinetic_energy_jacobi(self, x: torch.Tensor, **kwargs) -> torch.Tensor:
+        """-1/2 lap(psi)/psi by the Jacobi formula, (Nbatch, 1) (slater_jastrow.py:312-344)."""
+        return self._eval(x, N.QMC_EVAL_ELOC)["kinetic"].to(torch.get_default_dtype()).view(-1, 1)
+
+    def local_energy(self, pos: torch.Tensor) -> torch.Tensor:
+        """T + V_en + V_ee + V_nn, (Nbatch, 1) (wf_base.py:184-215)."""
+        return self._eval(pos, N.QMC_EVAL_ELOC)["eloc"].to(torch.get_default_dtype()).view(-1, 1)
+
+    def local_energy_parts(self, pos: torch.Tensor, precision: Optional[str] = None):
+        """All outputs of one local-energy evaluation (float64 dict)."""
+        return self._eval(pos, N.QMC_EVAL_ELOC, precision)
+
+    def nuclear_repulsion(self) -> float:
+        """V_nn (wf_base.py:97-116)."""
+        return self.device_plan().tables.vnn
+
+    def gradients_jacobi(self, x, sum_grad=False, pdf=False):
+        raise NotImplementedError("gradients_jacobi is not provided by the device path yet")
+
+    def log_data(self) -> None:
+        log.info("")
+        log.info(" Wave Function")
+        log.info("  Jastrow factor      : {0}", self.use_jastrow)
+        if self.use_jastrow:
+            log.info("  Jastrow kernel      : {0}", self.jastrow_type)
+        log.info("  Highest MO included : {0}", self.nmo_opt)
+        log.info("  Configurations      : {0}", self.configs_method)
+        log.info("  Number of confs     : {0}", self.nci)
+        log.info("  Kinetic energy      : {0}", self.kinetic_method)
+        log.info("  Precision           : {0}", self.precision)
+        log.info("  Number var  param   : {0}", self.get_number_parameters())
+        log.info("  Cuda support        : {0}", self.cuda)
+
+    def geometry(self, pos=None, convert_to_angs: Optional[bool] = False):
+        from ..utils.constants import BOHR2ANGS
+
+        conv = BOHR2ANGS if convert_to_angs else 1.0
+        return [(self.ao.atom_coords[i].detach().cpu().numpy() * conv).tolist() for i in range(self.natom)]
+
+
+class _PsiFunction(torch.autograd.Function):
+    """psi(pos; theta) with a hand-written adjoint for theta.
+
+    backward: with g_w = dL/dpsi_w * psi_w and the adjoint rows of
+    qmc_logpsi_grad_rows,
+        dL/dW       = AO_rows^T (g * Gamma_rows)       (one GEMM)
+        dL/dmo_mod  = dL/dW * mo_scf                   (molecular_orbitals.py:92)
+        dL/dc       = g^T (D_n / S)
+        dL/dw_ee/en = g^T djas
+        dL/dbas_exp = g^T dexp_rows
+    AO values are recomputed in the backward, nothing is stored by the forward.
+    Gradients with respect to the positions are not produced."""
+
+    @staticmethod
+    def forward(ctx, wf, x, mo_mod, ci, bas_exp, w_ee, w_en):
+        out = wf._eval(x, N.QMC_EVAL_PSI)
+        psi = out["psi"]
+        ctx.wf = wf
+        ctx.save_for_backward(x.detach(), psi)
+        return psi.to(torch.get_default_dtype()).view(-1, 1)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        wf = ctx.wf
+        x, psi = ctx.saved_tensors
+        g = grad_out.reshape(-1).to(torch.float64) * psi
+        rows = wf_adjoint_rows(wf, x, need_exp=ctx.needs_input_grad[4])
+        nel = wf.nelec
+        gr = g.repeat_interleave(nel)
+        dW = rows["ao"].T @ (rows["gamma"] * gr[:, None])  # (nao, nmo_used)
+        full = torch.zeros_like(wf.mo.mo_modifier, dtype=torch.float64)
+        full[:, : dW.shape[1]] = dW
+        d_mod = (full * wf.mo.mo_scf.to(full)).to(wf.mo.mo_modifier.dtype) if ctx.needs_input_grad[2] else None
+        d_ci = (g @ rows["dci"]).view(1, -1).to(wf.fc.weight.dtype) if ctx.needs_input_grad[3] else None
+        d_exp = None
+        if ctx.needs_input_grad[4]:
+            d_exp = (gr @ rows["dexp"]).to(wf.ao.bas_exp.dtype)
+        d_ee = (g @ rows["djas"][:, 0]).view(1).to(torch.get_default_dtype()) if ctx.needs_input_grad[5] else None
+        d_en = (g @ rows["djas"][:, 1]).view(1).to(torch.get_default_dtype()) if ctx.needs_input_grad[6] else None
+        return None, None, d_mod, d_ci, d_exp, d_ee, d_en
+
+
+def wf_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_exp: bool = False):
+    """Per-walker/per-electron adjoint rows of log|psi| (qmc_logpsi_grad_rows)."""
+    x = wf._check_pos(pos)
+    nw, nel = x.shape[0], wf.nelec
+    plan = wf.device_plan()
+    prm, keep = wf._params("fp64")
+    dev = x.device
+    f64 = torch.float64
+    nao = plan.tables.nao
+    rows = {
+        "ao": torch.empty(nw * nel, nao, dtype=f64, device=dev),
+        "gamma": torch.empty(nw * nel, plan.tables.nmo, dtype=f64, device=dev),
+        "dci": torch.empty(nw, plan.tables.nci, dtype=f64, device=dev),
+        "djas": torch.empty(nw, 2, dtype=f64, device=dev),
+        "dexp": torch.empty(nw * nel, plan.tables.nprim_ref, dtype=f64, device=dev) if need_exp else None,
+    }
+    N.check(N.load().qmc_logpsi_grad_rows(plan.handle, prm, x.data_ptr(), nw, rows["ao"].data_ptr(),
+                                          rows["gamma"].data_ptr(), rows["dci"].data_ptr(),
+                                          rows["djas"].data_ptr(), N.ptr(rows["dexp"]), N.current_stream()))
+    return rows

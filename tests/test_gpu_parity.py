@@ -1,0 +1,237 @@
+"""Parity of the HIP path with the oracle (needs an MI355X).
+
+Tolerances (FP64 path): psi and local energies within 1e-10 relative to the
+oracle on the same positions (the bar stated for the FP64 path); AO values
+within 1e-12.  The mixed-precision path is measured against the FP64 path.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import vmc_oracle as O
+from tests import helpers, systems
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(autouse=True)
+def _double():
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
+
+
+@pytest.mark.parametrize("which", ["h2o", "lih_adf"])
+def test_ao_values_and_derivatives(which):
+    _say("case ao", which)
+    mol = systems.h2o() if which == "h2o" else systems.lih_adf()
+    wf, orc = helpers.make_pair(mol, w_ee=None)
+    pos = helpers.positions(mol, 37, seed=2)
+    ao, dao, d2ao = wf.ao(pos.to(DEV), derivative=[0, 1, 2], sum_grad=False)
+    oa, od, ol = orc.ao(pos.clone().requires_grad_(), "all")
+    torch.testing.assert_close(ao.cpu(), oa.detach(), rtol=1e-12, atol=1e-13)
+    torch.testing.assert_close(dao.cpu(), od.detach(), rtol=1e-11, atol=1e-12)
+    torch.testing.assert_close(d2ao.cpu(), ol.detach(), rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("case", list(helpers.CASES))
+def test_psi_and_local_energy_fp64(case):
+    _say("case fp64", case)
+    mol, kw = helpers.CASES[case]()
+    wf, orc = helpers.make_pair(mol, **kw)
+    n = 23 if mol.nelec > 20 else 61
+    pos = helpers.positions(mol, n, seed=3)
+    psi = wf(pos.to(DEV)).detach().cpu()
+    psi_o = orc(pos).detach()
+    torch.testing.assert_close(psi, psi_o, rtol=1e-10, atol=1e-300)
+    parts = wf.local_energy_parts(pos.to(DEV))
+    el_o = orc.local_energy(pos).detach().view(-1)
+    ke_o = orc.kinetic_energy_jacobi(pos).detach().view(-1)
+    el = parts["eloc"].cpu()
+    ke = parts["kinetic"].cpu()
+    scale = el_o.abs().clamp(min=1.0)
+    _say("  max rel dE_L", float(((el - el_o).abs() / scale).max()))
+    assert float(((el - el_o).abs() / scale).max()) < 1e-10
+    assert float(((ke - ke_o).abs() / ke_o.abs().clamp(min=1.0)).max()) < 1e-10
+    pdf = wf.pdf(pos.to(DEV)).cpu()
+    torch.testing.assert_close(pdf, psi_o.view(-1) ** 2, rtol=1e-10, atol=1e-300)
+
+
+def test_lih_golden_local_energy_on_device():
+    """E_L of the reference's stored positions with the stored wavefunction state."""
+    _say("case golden LiH")
+    g = np.load(systems.GOLDEN + "/LiH_golden.npz")
+    mol = systems.lih_adf()
+    p = lambda k: g["SolverSlaterJastrow/wf/" + k]  # noqa: E731
+    from qmctorch_amd.wavefunction import SlaterJastrow
+
+    wf = SlaterJastrow(mol, configs="single_double(2,2)", cuda=True)
+    W = p("mo_scf/weight").T @ p("mo/weight").T
+    with torch.no_grad():
+        wf.mo.mo_scf = torch.as_tensor(W).to(DEV)
+        wf.fc.weight.copy_(torch.as_tensor(p("fc/weight")))
+        wf.jastrow.jastrow_kernel.weight.copy_(torch.as_tensor(p("jastrow/weight")))
+    el = wf.local_energy(torch.as_tensor(g["single_point/pos"]).to(DEV)).cpu().numpy()
+    ref = g["single_point/local_energy"]
+    _say("  max |dE_L| vs reference golden", np.abs(el - ref).max())
+    assert np.abs(el - ref).max() < 1e-9
+    assert abs(el.mean() - float(g["single_point/energy"])) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["lih_631g_cas22_ee_en", "h2o_ccpvdz_ee_en", "c6h6_ccpvdz_ee_en"])
+def test_mixed_precision_error(case):
+    _say("case mixed", case)
+    mol, kw = helpers.CASES[case]()
+    wf, _ = helpers.make_pair(mol, **kw)
+    pos = helpers.positions(mol, 512, seed=4).to(DEV)
+    e64 = wf.local_energy_parts(pos, "fp64")["eloc"]
+    e32 = wf.local_energy_parts(pos, "mixed")["eloc"]
+    d = (e32 - e64).abs() / mol.nelec
+    _say("  mixed-vs-fp64 |dE_L|/Ne  max %.3e  median %.3e  mean %.3e" % (d.max(), d.median(), d.mean()))
+    l64, _ = wf.log_psi(pos, "fp64")
+    l32, _ = wf.log_psi(pos, "mixed")
+    assert float((l64 - l32).abs().median()) < 1e-5
+    assert float(d.median()) < 1e-4
+
+
+def test_philox_device_matches_host_and_kat():
+    _say("case philox")
+    from qmctorch_amd import _native as N
+    from tests.test_native_abi import KAT, philox_host
+
+    rng = np.random.default_rng(0)
+    ins = [list(c) + list(k) for c, k, _ in KAT] + rng.integers(0, 2 ** 32, size=(61, 6)).tolist()
+    a = torch.tensor(np.asarray(ins, dtype=np.uint32).view(np.int32), device=DEV)
+    out = torch.empty(len(ins), 4, dtype=torch.int32, device=DEV)
+    N.check(N.load().qmc_philox4x32_10(a.data_ptr(), out.data_ptr(), len(ins), N.current_stream()))
+    got = out.cpu().numpy().view(np.uint32)
+    for i, row in enumerate(ins):
+        assert tuple(got[i]) == tuple(philox_host(row[:4], row[4:6]))
+    for i, (_, _, ref) in enumerate(KAT):
+        assert tuple(got[i]) == ref
+
+
+def _sampler(mol, nw, nstep, step=0.2, seed=11, init=None, ntherm=-1, ndecor=1):
+    from qmctorch_amd.sampler import Metropolis
+
+    return Metropolis(nwalkers=nw, nstep=nstep, step_size=step, ntherm=ntherm, ndecor=ndecor, nelec=mol.nelec,
+                      init=init or mol.domain("normal"), move={"type": "all-elec", "proba": "normal"}, cuda=True,
+                      seed=seed)
+
+
+def test_sweep_exact_eigenfunction():
+    """H atom, 1s STO zeta=1: every sampled configuration has E_L = -1/2."""
+    _say("case sweep exact")
+    from qmctorch_amd.wavefunction import SlaterJastrow
+
+    mol = systems.hydrogen_atom("sto", 1.0)
+    wf = SlaterJastrow(mol, jastrow=None, cuda=True)
+    s = _sampler(mol, 512, 200, step=0.5)
+    pos = s(wf.pdf, with_tqdm=False)
+    el = wf.local_energy(pos).view(-1)
+    assert torch.allclose(el, torch.full_like(el, -0.5), atol=1e-12)
+    assert 0.05 < s.last_acceptance < 0.999
+
+
+def test_sweep_gaussian_moments():
+    """|psi|^2 of a normalised s Gaussian (alpha = 0.5) gives <x^2> = 1/(4 alpha)."""
+    _say("case sweep moments")
+    from qmctorch_amd.wavefunction import SlaterJastrow
+
+    alpha = 0.5
+    mol = systems.hydrogen_atom("gto_pure", alpha)
+    wf = SlaterJastrow(mol, jastrow=None, cuda=True)
+    s = _sampler(mol, 4096, 400, step=1.0, ntherm=100, ndecor=20)
+    pos = s(wf.pdf, with_tqdm=False).detach()
+    x2 = (pos ** 2).mean(1)  # mean over x, y, z
+    m = float(x2.mean())
+    # independent walkers x 15 kept steps (decorrelated by 20 steps): conservative error bar
+    err = float(x2.std()) / math.sqrt(4096)
+    _say("  <x^2> = %.5f +- %.5f (exact %.5f)" % (m, err, 1 / (4 * alpha)))
+    assert abs(m - 1 / (4 * alpha)) < 6 * err
+
+
+def test_sweep_batch_invariance():
+    """A walker's chain depends on (seed, walker id, step) only, not on the batch."""
+    _say("case sweep batch invariance")
+    mol = systems.h2o()
+    wf, _ = helpers.make_pair(mol, w_ee=1.0, w_en=1.0, precision="mixed")
+    torch.manual_seed(0)
+    p0 = helpers.positions(mol, 96, seed=5).to(DEV)
+    a = _sampler(mol, 96, 30)(wf.pdf, pos=p0.clone(), with_tqdm=False).detach()
+    b = _sampler(mol, 40, 30)(wf.pdf, pos=p0[:40].clone(), with_tqdm=False).detach()
+    assert torch.equal(a[:40], b)
+
+
+def test_sweep_energy_matches_oracle_chain():
+    """H2/STO-3G: the device chain and the oracle's host-RNG chain sample the same
+    distribution (mean local energy within the combined standard error)."""
+    _say("case sweep vs oracle chain")
+    mol = systems.h2()
+    wf, orc = helpers.make_pair(mol, w_ee=1.0, perturb=0.0)
+    nw, nstep = 2000, 300
+    torch.manual_seed(0)
+    init = helpers.positions(mol, nw, seed=6)
+    pos_d = _sampler(mol, nw, nstep, ntherm=100, ndecor=40)(wf.pdf, pos=init.to(DEV).clone(), with_tqdm=False)
+    e_d = wf.local_energy(pos_d.detach()).view(-1).cpu()
+    pos_o, _ = O.metropolis(orc.pdf, init.clone(), nstep, step_size=0.2, ntherm=100, ndecor=40, seed=1)
+    e_o = orc.local_energy(pos_o).detach().view(-1)
+    se = math.sqrt(float(e_d.var()) / len(e_d) + float(e_o.var()) / len(e_o))
+    _say("  device %.5f  oracle %.5f  se %.5f" % (e_d.mean(), e_o.mean(), se))
+    assert abs(float(e_d.mean() - e_o.mean())) < 5 * se
+
+
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22"])
+def test_manual_gradient_adjoint_vs_oracle_autograd(case):
+    """psi.backward(2/N (E_L - <E_L>)/psi) (solver.py:372-431): adjoint kernel vs autograd."""
+    _say("case manual gradient", case)
+    mol, kw = helpers.CASES[case]()
+    wf, orc = helpers.make_pair(mol, **kw)
+    pos = helpers.positions(mol, 64, seed=8)
+    eloc_o, g_o = O.manual_gradient(orc, pos.clone().requires_grad_())
+    wf.zero_grad()
+    x = pos.to(DEV)
+    with torch.no_grad():
+        eloc = wf.local_energy(x)
+    psi = wf(x)
+    w = (eloc - eloc.mean()) / psi.detach() * (2.0 / len(psi))
+    psi.backward(w)
+    torch.testing.assert_close(wf.mo.mo_modifier.grad.cpu(), g_o["mo_modifier"], rtol=1e-8, atol=1e-10)
+    torch.testing.assert_close(wf.ao.bas_exp.grad.cpu(), g_o["bas_exp"], rtol=1e-8, atol=1e-10)
+    if g_o.get("fc_weight") is not None and wf.nci > 1:
+        torch.testing.assert_close(wf.fc.weight.grad.cpu(), g_o["fc_weight"], rtol=1e-8, atol=1e-10)
+    names = dict(wf.named_parameters())
+    for key_d, key_o in [("jastrow.jastrow_kernel.weight", "w_ee"),
+                         ("jastrow.jastrow_terms.0.jastrow_kernel.weight", "w_ee"),
+                         ("jastrow.jastrow_terms.1.jastrow_kernel.weight", "w_en")]:
+        if key_d in names and g_o.get(key_o) is not None:
+            torch.testing.assert_close(names[key_d].grad.cpu(), g_o[key_o], rtol=1e-8, atol=1e-10)
+
+
+def test_solver_single_point_and_run():
+    _say("case solver")
+    from qmctorch_amd.solver import Solver
+
+    mol = systems.h2()
+    wf, _ = helpers.make_pair(mol, w_ee=1.0, perturb=0.0)
+    s = _sampler(mol, 1000, 200)
+    opt = torch.optim.Adam(wf.parameters(), lr=0.01)
+    solver = Solver(wf=wf, sampler=s, optimizer=opt, output="/tmp/qmc_test_h2.hdf5")
+    obs = solver.single_point(with_tqdm=False)
+    assert obs.local_energy.shape == (1000, 1)
+    assert -1.4 < float(obs.energy) < -0.8
+    before = wf.mo.mo_modifier.detach().clone()
+    solver.configure(freeze=["ao"], track=["local_energy", "parameters"])
+    out = solver.run(3)
+    assert len(out.energy) == 4 and np.all(np.isfinite(out.energy))
+    assert not torch.equal(before, wf.mo.mo_modifier.detach())

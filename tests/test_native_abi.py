@@ -1,0 +1,96 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point declared in include/qmc_mi355x.h, and the ctypes structures match the
+C layout (checked by compiling the header with gcc)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from qmctorch_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "qmc_mi355x.h")
+
+
+def _declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qmc_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.isfile(N.LIBPATH):
+        pytest.skip("libqmcmi355x.so not built")
+    lib = N.load()
+    names = _declared_functions()
+    assert len(names) >= 8
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in N.EXPORTS, "binding missing for %s" % name
+    assert lib.qmc_version().decode().startswith("libqmcmi355x")
+
+
+def test_error_path_without_gpu_calls():
+    if not os.path.isfile(N.LIBPATH):
+        pytest.skip("libqmcmi355x.so not built")
+    lib = N.load()
+    rc = lib.qmc_plan_create(None, None)
+    assert rc == 1 and "null" in lib.qmc_last_error().decode()
+
+
+def test_ctypes_struct_layout_matches_header(tmp_path):
+    structs = {
+        "QmcSystemDesc": N.QmcSystemDesc,
+        "QmcParams": N.QmcParams,
+        "QmcWfOut": N.QmcWfOut,
+        "QmcWalkers": N.QmcWalkers,
+        "QmcSweepDesc": N.QmcSweepDesc,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "qmc_mi355x.h"', "int main(void) {"]
+    for sname, cls in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (sname, sname))
+        for fname, _ in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (sname, fname, sname, fname))
+    lines.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(c), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n") if l)
+    for sname, cls in structs.items():
+        assert int(out[sname]) == ctypes.sizeof(cls), sname
+        for fname, _ in cls._fields_:
+            assert int(out["%s.%s" % (sname, fname)]) == getattr(cls, fname).offset, (sname, fname)
+
+
+# ---------------------------------------------------------------- Philox
+M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+MASK = 0xFFFFFFFF
+
+
+def philox_host(ctr, key):
+    """Philox4x32-10 from Salmon et al., SC'11 (independent host restatement)."""
+    c = list(ctr)
+    k0, k1 = key
+    for _ in range(10):
+        p0 = M0 * c[0]
+        p1 = M1 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k0) & MASK, p1 & MASK, ((p0 >> 32) ^ c[3] ^ k1) & MASK, p0 & MASK]
+        k0 = (k0 + W0) & MASK
+        k1 = (k1 + W1) & MASK
+    return c
+
+
+KAT = [  # Random123 known-answer vectors for philox4x32-10
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((MASK, MASK, MASK, MASK), (MASK, MASK), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,ref", KAT)
+def test_philox_host_known_answers(ctr, key, ref):
+    assert tuple(philox_host(ctr, key)) == ref
