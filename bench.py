@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--steps-per-launch", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the VMC hot path on a single MI355X "
@@ -211,6 +213,13 @@ def main():
             "flops_per_walker_step": F + 4 * T,
         },
     }
+    if args.copy_calibration:
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        res["copy_calibration_bytes"] = {"read": src.numel() * 8, "write": dst.numel() * 8}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(mol, wf)
     print(json.dumps(res), flush=True)
