@@ -187,6 +187,23 @@ int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const dou
                          double* ao_rows, double* gamma_rows, double* dci, double* djas, double* dexp_rows,
                          qmc_stream_t stream);
 
+/* Adjoint of the local energy with respect to the parameters, as per-walker
+ * rows (FP64; AO values and derivatives are recomputed):
+ *   eloc    [nw]              local energy (or NULL)
+ *   ao_rows [nw*nelec][nao]   AO values          } dE_L/dW = ao_rows^T (g*ga_rows)
+ *   b_rows  [nw*nelec][nao]   AO-level kinetic   }         + b_rows^T (g*gb_rows)
+ *                             rows -1/2(lap AO + 2 grad lnJ . grad AO + (lap J/J) AO)
+ *   ga_rows, gb_rows [nw*nelec][nmo]
+ *   dci     [nw][nci]         dE_L/dc_n
+ *   djas    [nw][2]           dE_L/d(w_ee, w_en)
+ *   drift   [nw][nelec][3]    grad_i psi / psi (slater_jastrow.py:346-447) or NULL
+ * ao_rows and b_rows are both given or both NULL.  Replaces autograd through
+ * WaveFunction.local_energy in Solver.evaluate_grad_auto (solver.py:352-370,
+ * loss.py:49-79) and SlaterJastrow.gradients_jacobi. */
+int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw, double* eloc,
+                       double* ao_rows, double* b_rows, double* ga_rows, double* gb_rows, double* dci,
+                       double* djas, double* drift, qmc_stream_t stream);
+
 /* Philox4x32-10 of the sweep kernel applied to n (counter[4], key[2]) tuples
  * (device pointers: in [n*6] uint32, out [n*4] uint32).  No counterpart in the
  * reference (whose proposals come from torch's host generator); exposed so the

@@ -89,6 +89,8 @@ EXPORTS = {
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_void_p],
                              ctypes.c_int),
+    "qmc_eloc_grad_rows": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64]
+                           + [ctypes.c_void_p] * 8 + [ctypes.c_void_p], ctypes.c_int),
     "qmc_philox4x32_10": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "qmc_last_error": ([], ctypes.c_char_p),
     "qmc_version": ([], ctypes.c_char_p),

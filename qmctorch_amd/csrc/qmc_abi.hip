@@ -256,4 +256,34 @@ int qmc_philox4x32_10(const uint32_t* in, uint32_t* out, int64_t n, qmc_stream_t
     return QMC_OK;
 }
 
+int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw, double* eloc,
+                       double* ao_rows, double* b_rows, double* ga_rows, double* gb_rows, double* dci,
+                       double* djas, double* drift, qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!pos) return fail(QMC_ERR_ARG, "null pos");
+    if ((ao_rows == nullptr) != (b_rows == nullptr)) return fail(QMC_ERR_ARG, "ao_rows and b_rows go together");
+    if (nw == 0) return QMC_OK;
+    EArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.eloc = eloc;
+    a.ao_rows = ao_rows;
+    a.b_rows = b_rows;
+    a.ga_rows = ga_rows;
+    a.gb_rows = gb_rows;
+    a.dci = dci;
+    a.djas = djas;
+    a.drift = drift;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)nw * plan->t.nelec * plan->t.nao;
+    fill_zero(ao_rows, n, s);
+    fill_zero(b_rows, n, s);
+    if (launch_eloc_grad(a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
 }  // extern "C"

@@ -40,4 +40,25 @@ int launch_ao(const AArgs& a, bool deriv, hipStream_t s) {
     return 0;
 }
 
+template <int NM, int RAD>
+static void go_e(const EArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((eloc_grad_kernel<NM, RAD>), grid_for(a.t, a.nw), dim3(kBlock), 0, s, a);
+}
+
+template <int RAD>
+static int bucket_e(const EArgs& a, hipStream_t s) {
+    switch (nm_bucket(a.t)) {
+        case 2: go_e<2, RAD>(a, s); return 0;
+        case 4: go_e<4, RAD>(a, s); return 0;
+        case 8: go_e<8, RAD>(a, s); return 0;
+        case 16: go_e<16, RAD>(a, s); return 0;
+        case 24: go_e<24, RAD>(a, s); return 0;
+        default: return -1;
+    }
+}
+
+int launch_eloc_grad(const EArgs& a, hipStream_t s) {
+    return a.t.radial == QMC_RADIAL_STO ? bucket_e<QMC_RADIAL_STO>(a, s) : bucket_e<QMC_RADIAL_GTO_PURE>(a, s);
+}
+
 }  // namespace qmc

@@ -684,6 +684,350 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
     }
 }
 
+// ----------------------------------------------------------- local-energy adjoint
+// Rows for d E_L / d theta (qmc_eloc_grad_rows).  With w_n = c_n D_n / S,
+// t_n = sum_sigma tr(A^-1 B), T = sum_n w_n t_n (slater_jastrow.py:312-344):
+//   dT/dW[mu,m] = sum_i AO_i[mu] GA_i[m] + b_i[mu] GB_i[m]
+//     GA_i[m] = sum_c sum_{k: conf_c[k]=m} (rho_c A_c^-1 - omega_c A_c^-1 B_c A_c^-1)[k,i]
+//     GB_i[m] = sum_c omega_c sum_{k: conf_c[k]=m} (A_c^-1)[k,i]
+//     rho_c = sum_{n uses c} w_n (t_n - T), omega_c = sum_{n uses c} w_n,
+//     b_i[mu] = -1/2 (lap AO + 2 g.grad AO + (l + |g|^2) AO)  (the AO-level B row)
+//   dT/dw = -sum_i [dg_i/dw . (s_i + g_i) + 1/2 dl_i/dw],  s_i = grad_i S / S
+//   dT/dc_n = D_n (t_n - T) / S
+// where g_i, l_i are the gradient and Laplacian of ln J for electron i.
+struct EArgs {
+    Tables t;
+    QmcParams p;
+    const double* pos;
+    int64_t nw;
+    double* eloc;      // [nw] or null
+    double* ao_rows;   // [nw*nelec][nao] or null
+    double* b_rows;    // [nw*nelec][nao] or null
+    double* ga_rows;   // [nw*nelec][nmo] or null
+    double* gb_rows;   // [nw*nelec][nmo] or null
+    double* dci;       // [nw][nci] or null
+    double* djas;      // [nw][2] or null
+    double* drift;     // [nw][nelec][3] or null
+};
+
+// Jastrow gradient/Laplacian of ln J for one electron and their derivatives
+// with respect to the two Pade weights.
+__device__ __forceinline__ void jastrow_w_derivs(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                                 double y, double z, double (&g)[3], double& l, double (&dge)[3],
+                                                 double& dle, double (&dgn)[3], double& dln) {
+    for (int d = 0; d < 3; ++d) g[d] = dge[d] = dgn[d] = 0.0;
+    l = dle = dln = 0.0;
+    const double wee = (t.use_ee && prm.w_ee) ? prm.w_ee[0] : 0.0;
+    const double wen = (t.use_en && prm.w_en) ? prm.w_en[0] : 0.0;
+    for (int j = 0; j < t.nelec; ++j) {
+        const double xj = __shfl(x, L.wbase + j), yj = __shfl(y, L.wbase + j), zj = __shfl(z, L.wbase + j);
+        if (t.use_ee && j != L.e) {
+            const double dv[3] = {x - xj, y - yj, z - zj};
+            const double r = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            const double ir = 1.0 / r;
+            const double w0 = ((j < t.nup) == (L.e < t.nup)) ? 0.25 : 0.5;
+            const double q = 1.0 / (1.0 + wee * r);
+            const double kp = w0 * q * q, kpp = -2.0 * w0 * wee * q * q * q;
+            const double dkp = -2.0 * w0 * r * q * q * q;
+            const double dkpp = -2.0 * w0 * (1.0 - 2.0 * wee * r) * q * q * q * q;
+            for (int d = 0; d < 3; ++d) {
+                g[d] += kp * dv[d] * ir;
+                dge[d] += dkp * dv[d] * ir;
+            }
+            l += kpp + 2.0 * kp * ir;
+            dle += dkpp + 2.0 * dkp * ir;
+        }
+    }
+    if (t.use_en) {
+        for (int A = 0; A < t.natom; ++A) {
+            const double dv[3] = {x - t.jas_atoms[3 * A], y - t.jas_atoms[3 * A + 1], z - t.jas_atoms[3 * A + 2]};
+            const double r = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+            const double ir = 1.0 / r;
+            const double q = 1.0 / (1.0 + wen * r);
+            const double kp = q * q, kpp = -2.0 * wen * q * q * q;
+            const double dkp = -2.0 * r * q * q * q;
+            const double dkpp = -2.0 * (1.0 - 2.0 * wen * r) * q * q * q * q;
+            for (int d = 0; d < 3; ++d) {
+                g[d] += kp * dv[d] * ir;
+                dgn[d] += dkp * dv[d] * ir;
+            }
+            l += kpp + 2.0 * kp * ir;
+            dln += dkpp + 2.0 * dkp * ir;
+        }
+    }
+}
+
+// For this lane: column myrow of A^-1 (inv[k]) and of A^-1 B A^-1 (zz[k]), k < n,
+// from the two eliminations [A|B] (rows of X = A^-1 B in xb) and [A|I] (rows of
+// A^-1 in ib); col is this lane's pivot column (identical in both runs).
+template <int NM>
+__device__ __forceinline__ void inverse_columns(const WalkerLane& L, int nmax, int col, const double (&xb)[NM],
+                                                const double (&ib)[NM], double (&inv)[NM], double (&zz)[NM]) {
+#pragma unroll
+    for (int k = 0; k < NM; ++k) inv[k] = zz[k] = 0.0;
+    int srcs[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        int src = 0;
+        if (k < nmax) {
+            for (int s = 0; s < nmax; ++s) {
+                const int cs = __shfl(col, L.gbase + (s < L.n_sig ? s : 0));
+                if (s < L.n_sig && cs == k) src = s;
+            }
+        }
+        srcs[k] = src;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        if (k < nmax) {
+            double v = 0.0;
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                if (j < nmax) {
+                    const double vj = __shfl(ib[j], L.gbase + srcs[k]);
+                    if (j == L.myrow) v = vj;
+                }
+            }
+            inv[k] = (k < L.n_sig) ? v : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        if (k < nmax) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                if (j < nmax) {
+                    const double xkj = __shfl(xb[j], L.gbase + srcs[k]);
+                    s += xkj * inv[j];
+                }
+            }
+            zz[k] = (k < L.n_sig) ? s : 0.0;
+        }
+    }
+}
+
+template <int NM, int RAD>
+__global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
+    const Tables& t = a.t;
+    const WalkerLane L = lane_layout(t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
+    const double x = p[0], y = p[1], z = p[2];
+    const size_t row = (size_t)L.wclamp * t.nelec + L.e;
+    const bool v = L.valid;
+
+    double g[3], l, dge[3], dle, dgn[3], dln;
+    jastrow_w_derivs(t, a.p, L, x, y, z, g, l, dge, dle, dgn, dln);
+    const double l2 = l + g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+
+    double mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = 0.0;
+    {
+        const double* W = (const double*)a.p.mo_w;
+        double* aor = a.ao_rows ? a.ao_rows + row * t.nao : nullptr;
+        double* bor = a.b_rows ? a.b_rows + row * t.nao : nullptr;
+        auto sink = [&](int mu, double ao, double ax, double ay, double az, double al) {
+            if (v && aor) aor[mu] += ao;
+            if (v && bor) bor[mu] += -0.5 * (al + 2.0 * (g[0] * ax + g[1] * ay + g[2] * az) + l2 * ao);
+            const double* wr = W + (size_t)mu * t.nmo;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                if (m < t.nmo) {
+                    const double w = wr[m];
+                    mo[m] += ao * w;
+                    gx[m] += ax * w;
+                    gy[m] += ay * w;
+                    gz[m] += az * w;
+                    lp[m] += al * w;
+                }
+            }
+        };
+        ao_row<double, RAD, true>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+    }
+    double bk[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+        bk[m] = -0.5 * (lp[m] + 2.0 * (g[0] * gx[m] + g[1] * gy[m] + g[2] * gz[m]) + l2 * mo[m]);
+
+    // pass 1: log|det|, sign and trace of every unique configuration
+    double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        ld[c] = 0.0;
+        sd[c] = 1.0;
+        td[c] = 0.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < t.nconf_max; ++c) {
+        double aa[NM], bb[NM];
+        gather_conf<NM>(t, L, c, mo, aa);
+        gather_conf<NM>(t, L, c, bk, bb);
+        double lg, sg, tp;
+        int col;
+        lane_gauss_jordan<NM, true>(aa, bb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        const double tr = seg_sum(tp, L.gbase, L.n_sig, t.nsig_max);
+#pragma unroll
+        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+            if (c2 == c) {
+                ld[c2] = lg;
+                sd[c2] = sg;
+                td[c2] = tr;
+            }
+    }
+    // walker-level CI combination
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    double lu[QMC_MAXCONF], su[QMC_MAXCONF], tu[QMC_MAXCONF], lo[QMC_MAXCONF], so[QMC_MAXCONF], to[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) {
+        lu[c] = __shfl(ld[c], up_lane);
+        su[c] = __shfl(sd[c], up_lane);
+        tu[c] = __shfl(td[c], up_lane);
+        const double l_ = __shfl(ld[c], dn_lane), s_ = __shfl(sd[c], dn_lane), t_ = __shfl(td[c], dn_lane);
+        lo[c] = t.ndown > 0 ? l_ : 0.0;
+        so[c] = t.ndown > 0 ? s_ : 1.0;
+        to[c] = t.ndown > 0 ? t_ : 0.0;
+    }
+    auto term = [&](int n, double& lg, double& sg, double& tr) {
+        lg = 0.0;
+        sg = 1.0;
+        tr = 0.0;
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            if (c == t.ci_up[n]) {
+                lg += lu[c];
+                sg *= su[c];
+                tr += tu[c];
+            }
+            if (c == t.ci_dn[n]) {
+                lg += lo[c];
+                sg *= so[c];
+                tr += to[c];
+            }
+        }
+    };
+    double lmax = -INFINITY;
+    for (int n = 0; n < t.nci; ++n) {
+        double lg, sg, tr;
+        term(n, lg, sg, tr);
+        lmax = fmax(lmax, lg);
+    }
+    if (!(lmax > -INFINITY)) lmax = 0.0;
+    double S = 0.0, ST = 0.0;
+    for (int n = 0; n < t.nci; ++n) {
+        double lg, sg, tr;
+        term(n, lg, sg, tr);
+        const double dn = a.p.ci[n] * sg * exp(lg - lmax);
+        S += dn;
+        ST += dn * tr;
+    }
+    const double T = ST / S;
+    double rho[QMC_MAXCONF], om[QMC_MAXCONF];
+#pragma unroll
+    for (int c = 0; c < QMC_MAXCONF; ++c) rho[c] = om[c] = 0.0;
+    for (int n = 0; n < t.nci; ++n) {
+        double lg, sg, tr;
+        term(n, lg, sg, tr);
+        const double dns = sg * exp(lg - lmax) / S;  // D_n / S
+        const double wn = a.p.ci[n] * dns;
+        const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c)
+            if (c == mine) {
+                rho[c] += wn * (tr - T);
+                om[c] += wn;
+            }
+        if (v && L.e == 0 && a.dci) a.dci[(size_t)L.w * t.nci + n] = dns * (tr - T);
+    }
+    // pass 2: inverse columns, adjoint rows and the determinant drift s_i
+    double GA[NM], GB[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) GA[m] = GB[m] = 0.0;
+    double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int c = 0; c < t.nconf_max; ++c) {
+        double aa[NM], xb[NM], ib[NM];
+        gather_conf<NM>(t, L, c, mo, aa);
+        gather_conf<NM>(t, L, c, bk, xb);
+        double lg, sg, tp;
+        int col, col2;
+        lane_gauss_jordan<NM, true>(aa, xb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        gather_conf<NM>(t, L, c, mo, aa);
+#pragma unroll
+        for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
+        lane_gauss_jordan<NM, true>(aa, ib, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col2);
+        double inv[NM], zz[NM];
+        inverse_columns<NM>(L, t.nsig_max, col, xb, ib, inv, zz);
+        double rc = 0.0, oc = 0.0;
+#pragma unroll
+        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+            if (c2 == c) {
+                rc = rho[c2];
+                oc = om[c2];
+            }
+        const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+        const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
+        if (c < nconf) {
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                if (k < L.n_sig) {
+                    const int mcol = t.identity_conf ? k : conf[k];
+#pragma unroll
+                    for (int m = 0; m < NM; ++m) {
+                        if (m == mcol) {
+                            GA[m] += rc * inv[k] - oc * zz[k];
+                            GB[m] += oc * inv[k];
+                            s[0] += oc * inv[k] * gx[m];
+                            s[1] += oc * inv[k] * gy[m];
+                            s[2] += oc * inv[k] * gz[m];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (v) {
+        if (a.ga_rows) {
+            double* o = a.ga_rows + row * t.nmo;
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m < t.nmo) o[m] = GA[m];
+        }
+        if (a.gb_rows) {
+            double* o = a.gb_rows + row * t.nmo;
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+                if (m < t.nmo) o[m] = GB[m];
+        }
+        if (a.drift) {
+            double* o = a.drift + row * 3;
+            o[0] = s[0] + g[0];
+            o[1] = s[1] + g[1];
+            o[2] = s[2] + g[2];
+        }
+    }
+    double dte = 0.0, dtn = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        dte -= dge[d] * (s[d] + g[d]);
+        dtn -= dgn[d] * (s[d] + g[d]);
+    }
+    dte -= 0.5 * dle;
+    dtn -= 0.5 * dln;
+    const double sdte = seg_sum(dte, L.wbase, t.nelec, t.nelec);
+    const double sdtn = seg_sum(dtn, L.wbase, t.nelec, t.nelec);
+    if (a.eloc) {
+        double kee, ken, d0, d1, d2, d3, vpot;
+        jastrow_and_potential<true>(t, a.p, L, x, y, z, kee, ken, d0, d1, d2, d3, vpot);
+        const double ep = seg_sum(vpot, L.wbase, t.nelec, t.nelec) + t.vnn;
+        if (v && L.e == 0) a.eloc[L.w] = T + ep;
+    }
+    if (v && L.e == 0 && a.djas) {
+        a.djas[(size_t)L.w * 2 + 0] = sdte;
+        a.djas[(size_t)L.w * 2 + 1] = sdtn;
+    }
+}
+
 // Zero fill, a kernel rather than a memset node so that the entry points stay
 // safe to record in HIP graphs.
 static __global__ __launch_bounds__(kBlock) void fill_zero_kernel(double* p, size_t n) {

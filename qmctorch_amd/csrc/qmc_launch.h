@@ -29,5 +29,6 @@ int launch_sweep_f32(const SArgs& a, hipStream_t s);
 int launch_sweep_f64(const SArgs& a, hipStream_t s);
 int launch_grad(const GArgs& a, hipStream_t s);
 int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
+int launch_eloc_grad(const EArgs& a, hipStream_t s);
 
 }  // namespace qmc

@@ -232,8 +232,25 @@ class SlaterJastrow(WaveFunction):
         """V_nn (wf_base.py:97-116)."""
         return self.device_plan().tables.vnn
 
-    def gradients_jacobi(self, x, sum_grad=False, pdf=False):
-        raise NotImplementedError("gradients_jacobi is not provided by the device path yet")
+    def local_energy_autodiff(self, pos: torch.Tensor) -> torch.Tensor:
+        """Local energy differentiable with respect to the parameters through the
+        hand-written adjoint (qmc_eloc_grad_rows); used by grad='auto'."""
+        ee, en = self._jastrow_terms()
+        params = [self.mo.mo_modifier, self.fc.weight, self.ao.bas_exp,
+                  ee.jastrow_kernel.weight if ee is not None else None,
+                  en.jastrow_kernel.weight if en is not None else None]
+        return _ELocFunction.apply(self, pos, *params)
+
+    def gradients_jacobi(self, x: torch.Tensor, sum_grad: Optional[bool] = False,
+                         pdf: Optional[bool] = False) -> torch.Tensor:
+        """grad psi (or grad psi^2 with pdf=True), shape (Nbatch, 3 Nelec)
+        (slater_jastrow.py:346-447): psi * (grad ln J + grad S / S)."""
+        rows = eloc_adjoint_rows(self, x, need_rows=False)
+        psi = self._eval(x, N.QMC_EVAL_PSI, "fp64")["psi"]
+        out = rows["drift"].view(x.shape[0], -1) * psi[:, None]
+        if pdf:
+            out = 2.0 * out * psi[:, None]
+        return out.to(torch.get_default_dtype())
 
     def log_data(self) -> None:
         log.info("")
@@ -318,3 +335,65 @@ def wf_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_exp: bool = False
                                           rows["gamma"].data_ptr(), rows["dci"].data_ptr(),
                                           rows["djas"].data_ptr(), N.ptr(rows["dexp"]), N.current_stream()))
     return rows
+
+
+class _ELocFunction(torch.autograd.Function):
+    """E_L(pos; theta) with the hand-written adjoint of qmc_eloc_grad_rows:
+        dL/dW    = ao_rows^T (g * ga_rows) + b_rows^T (g * gb_rows)
+        dL/dc    = g^T dci,   dL/dw_ee,en = g^T djas
+    (g = dL/dE_L per walker).  Exponent gradients of E_L are not provided."""
+
+    @staticmethod
+    def forward(ctx, wf, x, mo_mod, ci, bas_exp, w_ee, w_en):
+        el = wf._eval(x, N.QMC_EVAL_ELOC)["eloc"]
+        ctx.wf = wf
+        ctx.save_for_backward(x.detach())
+        return el.to(torch.get_default_dtype()).view(-1, 1)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        wf = ctx.wf
+        (x,) = ctx.saved_tensors
+        if ctx.needs_input_grad[4]:
+            raise NotImplementedError("d E_L / d bas_exp is not provided by the device adjoint: "
+                                      "freeze 'ao' (Solver.configure(freeze=['ao']))")
+        g = grad_out.reshape(-1).to(torch.float64)
+        r = eloc_adjoint_rows(wf, x, need_rows=ctx.needs_input_grad[2])
+        gr = g.repeat_interleave(wf.nelec)
+        d_mod = d_ci = d_ee = d_en = None
+        if ctx.needs_input_grad[2]:
+            dW = r["ao"].T @ (r["ga"] * gr[:, None]) + r["b"].T @ (r["gb"] * gr[:, None])
+            full = torch.zeros_like(wf.mo.mo_modifier, dtype=torch.float64)
+            full[:, : dW.shape[1]] = dW
+            d_mod = (full * wf.mo.mo_scf.to(full)).to(wf.mo.mo_modifier.dtype)
+        if ctx.needs_input_grad[3]:
+            d_ci = (g @ r["dci"]).view(1, -1).to(wf.fc.weight.dtype)
+        if ctx.needs_input_grad[5]:
+            d_ee = (g @ r["djas"][:, 0]).view(1).to(torch.get_default_dtype())
+        if ctx.needs_input_grad[6]:
+            d_en = (g @ r["djas"][:, 1]).view(1).to(torch.get_default_dtype())
+        return None, None, d_mod, d_ci, None, d_ee, d_en
+
+
+def eloc_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_rows: bool = True):
+    """Rows of the local-energy adjoint (qmc_eloc_grad_rows), FP64."""
+    x = wf._check_pos(pos)
+    nw, nel = x.shape[0], wf.nelec
+    plan = wf.device_plan()
+    prm, keep = wf._params("fp64")
+    dev, f64 = x.device, torch.float64
+    nao, nmo = plan.tables.nao, plan.tables.nmo
+    r = {
+        "eloc": torch.empty(nw, dtype=f64, device=dev),
+        "ao": torch.empty(nw * nel, nao, dtype=f64, device=dev) if need_rows else None,
+        "b": torch.empty(nw * nel, nao, dtype=f64, device=dev) if need_rows else None,
+        "ga": torch.empty(nw * nel, nmo, dtype=f64, device=dev) if need_rows else None,
+        "gb": torch.empty(nw * nel, nmo, dtype=f64, device=dev) if need_rows else None,
+        "dci": torch.empty(nw, plan.tables.nci, dtype=f64, device=dev),
+        "djas": torch.empty(nw, 2, dtype=f64, device=dev),
+        "drift": torch.empty(nw, nel, 3, dtype=f64, device=dev),
+    }
+    N.check(N.load().qmc_eloc_grad_rows(plan.handle, prm, x.data_ptr(), nw, N.ptr(r["eloc"]), N.ptr(r["ao"]),
+                                        N.ptr(r["b"]), N.ptr(r["ga"]), N.ptr(r["gb"]), N.ptr(r["dci"]),
+                                        N.ptr(r["djas"]), N.ptr(r["drift"]), N.current_stream()))
+    return r
