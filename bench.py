@@ -90,7 +90,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the VMC hot path on a single MI355X "
-              "(walkers shard trivially across GPUs, see DESIGN.md); --gpus %d is not supported" % args.gpus,
+              "(walkers shard trivially across GPUs); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
