@@ -17,8 +17,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libqmcmi355x.so")
-SOURCES = ["qmc_abi.hip", "qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip",
-           "qmc_grad.hip"]
+# (source, object tag, extra defines): kernel families are compiled once per radial type
+_FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip", "qmc_grad.hip"]
+UNITS = [("qmc_abi.hip", "qmc_abi", [])] + [
+    (f, f.replace(".hip", "_r%d" % r), ["-DQMC_RAD=%d" % r]) for f in _FAMILIES for r in (0, 1)
+]
+SOURCES = [u[0] for u in UNITS]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950",
@@ -38,24 +42,25 @@ def _obj_dir() -> str:
     return d
 
 
-def compile_one(src: str, extra=()) -> subprocess.CompletedProcess:
-    obj = os.path.join(_obj_dir(), src.replace(".hip", ".o"))
-    cmd = [HIPCC, *FLAGS, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+def compile_one(unit, extra=()) -> subprocess.CompletedProcess:
+    src, tag, defs = unit
+    obj = os.path.join(_obj_dir(), tag + ".o")
+    cmd = [HIPCC, *FLAGS, *defs, *extra, "-c", os.path.join(HERE, src), "-o", obj]
     return subprocess.run(cmd, capture_output=True, text=True)
 
 
 def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) -> str:
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    jobs = jobs or min(len(SOURCES), os.cpu_count() or 4, 16)
+    jobs = jobs or min(len(UNITS), os.cpu_count() or 4, 16)
     with ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(lambda s: compile_one(s, extra), SOURCES))
-    for src, r in zip(SOURCES, results):
+        results = list(ex.map(lambda u: compile_one(u, extra), UNITS))
+    for unit, r in zip(UNITS, results):
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError("hipcc failed on %s" % src)
+            raise RuntimeError("hipcc failed on %s" % unit[1])
         if verbose or resource_usage:
             sys.stderr.write(r.stderr)
-    objs = [os.path.join(_obj_dir(), s.replace(".hip", ".o")) for s in SOURCES]
+    objs = [os.path.join(_obj_dir(), u[1] + ".o") for u in UNITS]
     tmp = LIB + ".tmp%d" % os.getpid()
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

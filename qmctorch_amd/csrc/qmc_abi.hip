@@ -116,6 +116,11 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.gao_mu = p->upload(d->gao_mu, d->nentry);
         t.gao_k = p->upload(d->gao_k, d->nentry);
         t.gao_K = p->upload(d->gao_K, (size_t)d->nentry * QMC_MAXP);
+        {
+            std::vector<float> kf((size_t)d->nentry * QMC_MAXP);
+            for (size_t i = 0; i < kf.size(); ++i) kf[i] = (float)d->gao_K[i];
+            t.gao_Kf = p->upload(kf.data(), kf.size());
+        }
         t.gao_n = p->upload(d->gao_n, (size_t)d->nentry * QMC_MAXP);
         t.gao_src = p->upload(d->gao_src, (size_t)d->nentry * QMC_MAXP);
         t.atom_z = p->upload(d->atom_z, d->natom);
@@ -171,7 +176,8 @@ int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision,
     hipStream_t s = (hipStream_t)stream;
     const bool sto = plan->t.radial == QMC_RADIAL_STO;
     (void)sto;
-    rc = precision == QMC_PREC_FP64 ? launch_wf_f64(a, mode, s) : launch_wf_f32(a, mode, s);
+    rc = precision == QMC_PREC_FP64 ? QMC_BY_RAD(launch_wf_f64, a.t.radial, a, mode, s)
+                                    : QMC_BY_RAD(launch_wf_f32, a.t.radial, a, mode, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
@@ -193,7 +199,8 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
     hipStream_t s = (hipStream_t)stream;
     const bool sto = plan->t.radial == QMC_RADIAL_STO;
     (void)sto;
-    rc = precision == QMC_PREC_FP64 ? launch_sweep_f64(a, s) : launch_sweep_f32(a, s);
+    rc = precision == QMC_PREC_FP64 ? QMC_BY_RAD(launch_sweep_f64, a.t.radial, a, s)
+                                    : QMC_BY_RAD(launch_sweep_f32, a.t.radial, a, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
@@ -218,7 +225,7 @@ int qmc_ao_eval(const QmcPlan* plan, const QmcParams* params, const double* pos,
     fill_zero(ao, n, s);
     if (dao) fill_zero(dao, 3 * n, s);
     if (d2ao) fill_zero(d2ao, n, s);
-    launch_ao(a, dao || d2ao, s);
+    QMC_BY_RAD(launch_ao, a.t.radial, a, dao || d2ao, s);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }
@@ -243,7 +250,7 @@ int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const dou
     hipStream_t s = (hipStream_t)stream;
     fill_zero(ao_rows, (size_t)nw * plan->t.nelec * plan->t.nao, s);
     if (dexp_rows) fill_zero(dexp_rows, (size_t)nw * plan->t.nelec * plan->t.nprim_ref, s);
-    if (launch_grad(a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    if (QMC_BY_RAD(launch_grad, a.t.radial, a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }
@@ -281,7 +288,7 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
     const size_t n = (size_t)nw * plan->t.nelec * plan->t.nao;
     fill_zero(ao_rows, n, s);
     fill_zero(b_rows, n, s);
-    if (launch_eloc_grad(a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    if (QMC_BY_RAD(launch_eloc_grad, a.t.radial, a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }

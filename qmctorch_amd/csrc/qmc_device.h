@@ -26,6 +26,7 @@ struct Tables {
     const int* gao_mu;
     const int* gao_k;
     const double* gao_K;
+    const float* gao_Kf;
     const double* gao_n;
     const int* gao_src;
     const double* atom_z;
@@ -81,9 +82,87 @@ __device__ __forceinline__ void cart_poly(T x, T y, T z, int kx, int ky, int kz,
 //     AO = Y sum K R,  grad AO = (sum K R) grad Y + Y (sum K R q) r,
 //     lap AO = Y sum K R (n(n+1)/r^2 - 2a(n+1)/r + a^2 + 2Lq) + (sum K R) lap Y
 //
-// AOSINK(mu, ao, gx, gy, gz, lap) is called once per AO entry; entries of one
+// sink(mu, ao, gx, gy, gz, lap) is called once per AO entry; entries of one
 // AO that were split across groups are delivered separately (all consumers
-// are linear in the AO).
+// are linear in the AO).  The primitive count of a group selects a fully
+// unrolled instance (no predicated slots).
+template <typename T>
+__device__ __forceinline__ const T* coeff_table(const Tables& t);
+template <>
+__device__ __forceinline__ const float* coeff_table<float>(const Tables& t) {
+    return t.gao_Kf;
+}
+template <>
+__device__ __forceinline__ const double* coeff_table<double>(const Tables& t) {
+    return t.gao_K;
+}
+
+template <typename T, int RAD, bool DERIV, int NP, typename Sink>
+__device__ __forceinline__ void ao_group(const Tables& t, int g, const double* __restrict__ bas_exp, T dx, T dy,
+                                         T dz, T r2, T r, T ir, Sink& sink) {
+    const int p0 = t.grp_prim0[g];
+    T E[NP], A[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const T a = (T)bas_exp[t.grp_prim_src[p0 + p]];
+        A[p] = a;
+        E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
+    }
+    const T* __restrict__ Ktab = coeff_table<T>(t);
+    const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
+    for (int i = i0; i < i1; ++i) {
+        const int kk = t.gao_k[i];
+        const int kx = kk & 15, ky = (kk >> 4) & 15, kz = (kk >> 8) & 15;
+        T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
+        cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
+        const T* Kp = Ktab + (size_t)i * QMC_MAXP;
+        T s0 = 0, s1 = 0, s2 = 0;
+        if (RAD == QMC_RADIAL_GTO_PURE) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const T ke = Kp[p] * E[p];
+                s0 += ke;
+                if (DERIV) {
+                    s1 += ke * A[p];
+                    s2 += ke * A[p] * A[p];
+                }
+            }
+            const T ao = Y * s0;
+            if (DERIV) {
+                const T L = T(kx + ky + kz);
+                const T c = T(-2) * s1 * Y;
+                sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
+                     Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
+            } else {
+                sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+            }
+        } else {
+            const double* Np = t.gao_n + (size_t)i * QMC_MAXP;
+            const T L = T(kx + ky + kz);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const T n = (T)Np[p];
+                const int ni = (int)Np[p];
+                const T kr = Kp[p] * ipow(r, ni) * E[p];
+                s0 += kr;
+                if (DERIV) {
+                    const T a = A[p];
+                    const T q = n * ir * ir - a * ir;
+                    s1 += kr * q;
+                    s2 += kr * (n * (n + T(1)) * ir * ir - T(2) * a * (n + T(1)) * ir + a * a + T(2) * L * q);
+                }
+            }
+            const T ao = Y * s0;
+            if (DERIV) {
+                const T c = Y * s1;
+                sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
+            } else {
+                sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+            }
+        }
+    }
+}
+
 template <typename T, int RAD, bool DERIV, typename Sink>
 __device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict__ bas_exp,
                                        const double* __restrict__ atom_xyz, double ex, double ey, double ez,
@@ -103,75 +182,13 @@ __device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict
                 ir = T(1) / r;
             }
         }
-        const int np = t.grp_nprim[g];
-        const int p0 = t.grp_prim0[g];
-        T E[QMC_MAXP], A[QMC_MAXP];
-#pragma unroll
-        for (int p = 0; p < QMC_MAXP; ++p) {
-            if (p < np) {
-                const T a = (T)bas_exp[t.grp_prim_src[p0 + p]];
-                A[p] = a;
-                E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
-            } else {
-                A[p] = T(0);
-                E[p] = T(0);
-            }
-        }
-        const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
-        for (int i = i0; i < i1; ++i) {
-            const int kk = t.gao_k[i];
-            const int kx = kk & 15, ky = (kk >> 4) & 15, kz = (kk >> 8) & 15;
-            T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
-            cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
-            const double* Kp = t.gao_K + (size_t)i * QMC_MAXP;
-            T s0 = 0, s1 = 0, s2 = 0;
-            if (RAD == QMC_RADIAL_GTO_PURE) {
-#pragma unroll
-                for (int p = 0; p < QMC_MAXP; ++p) {
-                    if (p < np) {
-                        const T ke = (T)Kp[p] * E[p];
-                        s0 += ke;
-                        if (DERIV) {
-                            s1 += ke * A[p];
-                            s2 += ke * A[p] * A[p];
-                        }
-                    }
-                }
-                const T ao = Y * s0;
-                if (DERIV) {
-                    const T L = T(kx + ky + kz);
-                    const T c = T(-2) * s1 * Y;
-                    sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
-                         Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
-                } else {
-                    sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
-                }
-            } else {
-                const double* Np = t.gao_n + (size_t)i * QMC_MAXP;
-                const T L = T(kx + ky + kz);
-#pragma unroll
-                for (int p = 0; p < QMC_MAXP; ++p) {
-                    if (p < np) {
-                        const T n = (T)Np[p];
-                        const int ni = (int)Np[p];
-                        const T kr = (T)Kp[p] * ipow(r, ni) * E[p];
-                        s0 += kr;
-                        if (DERIV) {
-                            const T a = A[p];
-                            const T q = n * ir * ir - a * ir;
-                            s1 += kr * q;
-                            s2 += kr * (n * (n + T(1)) * ir * ir - T(2) * a * (n + T(1)) * ir + a * a + T(2) * L * q);
-                        }
-                    }
-                }
-                const T ao = Y * s0;
-                if (DERIV) {
-                    const T c = Y * s1;
-                    sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
-                } else {
-                    sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
-                }
-            }
+        switch (t.grp_nprim[g]) {
+#define QMC_AO_CASE(K) \
+    case K: ao_group<T, RAD, DERIV, K>(t, g, bas_exp, dx, dy, dz, r2, r, ir, sink); break;
+            QMC_AO_CASE(1) QMC_AO_CASE(2) QMC_AO_CASE(3) QMC_AO_CASE(4) QMC_AO_CASE(5) QMC_AO_CASE(6)
+            QMC_AO_CASE(7) QMC_AO_CASE(8) QMC_AO_CASE(9) QMC_AO_CASE(10) QMC_AO_CASE(11) QMC_AO_CASE(12)
+#undef QMC_AO_CASE
+            default: break;
         }
     }
 }
@@ -195,6 +212,8 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
     sgn = 1.0;
     int inversions = 0;
     bool singular = false;
+    double mant = 1.0;  // |det| = mant * 2^expo, renormalised after every pivot
+    int expo = 0;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
         if (k < nmax) {
@@ -217,7 +236,9 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
             const bool is_piv = colk && (myrow == br);
             if (colk) {
                 singular = singular || (piv == 0.0);
-                logdet += log(fabs(piv));
+                int ex;
+                mant = frexp(mant * fabs(piv), &ex);
+                expo += ex;
                 sgn *= (piv < 0.0) ? -1.0 : 1.0;
             }
             const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
@@ -245,7 +266,7 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
         }
     }
     if (inversions & 1) sgn = -sgn;
-    if (singular) logdet = -INFINITY;
+    logdet = singular ? -INFINITY : log(mant) + 0.6931471805599453 * (double)expo;
     trace_part = 0.0;
     if (WITH_B) {
 #pragma unroll

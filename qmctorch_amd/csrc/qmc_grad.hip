@@ -1,6 +1,10 @@
 // Adjoint-row and AO-layout kernels (FP64).
 #include "qmc_launch.h"
 
+#ifndef QMC_RAD
+#error "compile with -DQMC_RAD=0 (gto_pure) or -DQMC_RAD=1 (sto)"
+#endif
+
 namespace qmc {
 
 template <int NM, int RAD>
@@ -20,23 +24,18 @@ static int bucket(const GArgs& a, hipStream_t s) {
     }
 }
 
-int launch_grad(const GArgs& a, hipStream_t s) {
-    return a.t.radial == QMC_RADIAL_STO ? bucket<QMC_RADIAL_STO>(a, s) : bucket<QMC_RADIAL_GTO_PURE>(a, s);
+template <>
+int launch_grad<QMC_RAD>(const GArgs& a, hipStream_t s) {
+    return bucket<QMC_RAD>(a, s);
 }
 
-int launch_ao(const AArgs& a, bool deriv, hipStream_t s) {
+template <>
+int launch_ao<QMC_RAD>(const AArgs& a, bool deriv, hipStream_t s) {
     const dim3 g = grid_for(a.t, a.nw);
-    if (a.t.radial == QMC_RADIAL_STO) {
-        if (deriv)
-            hipLaunchKernelGGL((ao_kernel<QMC_RADIAL_STO, true>), g, dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((ao_kernel<QMC_RADIAL_STO, false>), g, dim3(kBlock), 0, s, a);
-    } else {
-        if (deriv)
-            hipLaunchKernelGGL((ao_kernel<QMC_RADIAL_GTO_PURE, true>), g, dim3(kBlock), 0, s, a);
-        else
-            hipLaunchKernelGGL((ao_kernel<QMC_RADIAL_GTO_PURE, false>), g, dim3(kBlock), 0, s, a);
-    }
+    if (deriv)
+        hipLaunchKernelGGL((ao_kernel<QMC_RAD, true>), g, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((ao_kernel<QMC_RAD, false>), g, dim3(kBlock), 0, s, a);
     return 0;
 }
 
@@ -57,8 +56,9 @@ static int bucket_e(const EArgs& a, hipStream_t s) {
     }
 }
 
-int launch_eloc_grad(const EArgs& a, hipStream_t s) {
-    return a.t.radial == QMC_RADIAL_STO ? bucket_e<QMC_RADIAL_STO>(a, s) : bucket_e<QMC_RADIAL_GTO_PURE>(a, s);
+template <>
+int launch_eloc_grad<QMC_RAD>(const EArgs& a, hipStream_t s) {
+    return bucket_e<QMC_RAD>(a, s);
 }
 
 }  // namespace qmc

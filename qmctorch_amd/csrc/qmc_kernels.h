@@ -137,6 +137,53 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
     }
 }
 
+// Value of ln J = 1/2 sum_{i != j} B_ee + sum_A B_en seen from one electron
+// (summed over the walker's lanes by the caller), in the bulk precision T.
+template <typename T>
+__device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                           double y, double z) {
+    T k = T(0);
+    if (t.use_ee) {
+        const T wee = (T)prm.w_ee[0];
+        const T xs = (T)x, ys = (T)y, zs = (T)z;
+        for (int j = 0; j < t.nelec; ++j) {
+            const T xj = __shfl(xs, L.wbase + j), yj = __shfl(ys, L.wbase + j), zj = __shfl(zs, L.wbase + j);
+            const T dx = xs - xj, dy = ys - yj, dz = zs - zj;
+            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+            const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
+            k += (j != L.e) ? w0 * r / (T(1) + wee * r) : T(0);
+        }
+    }
+    if (t.use_en) {
+        const T wen = (T)prm.w_en[0];
+        for (int A = 0; A < t.natom; ++A) {
+            const T dx = (T)(x - t.jas_atoms[3 * A]), dy = (T)(y - t.jas_atoms[3 * A + 1]),
+                    dz = (T)(z - t.jas_atoms[3 * A + 2]);
+            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+            k += r / (T(1) + wen * r);
+        }
+    }
+    return k;
+}
+
+// Single configuration, single CI term: log|D_up D_dn| and its sign (ground state).
+template <int NM, typename T>
+__device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                              const T (&mo)[NM], double& logabs, double& sgn) {
+    double a[NM], b[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) a[j] = (double)mo[j];
+    double lg, sg, tp;
+    int col;
+    lane_gauss_jordan<NM, false>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    const int dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
+    const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
+    const double c0 = prm.ci[0];
+    logabs = lu + (t.ndown > 0 ? ldn : 0.0) + log(fabs(c0));
+    sgn = su * (t.ndown > 0 ? sdn : 1.0) * (c0 < 0.0 ? -1.0 : 1.0);
+}
+
 // Gather row entries of configuration c from an MO-like row (uniform c).
 template <int NM, typename S>
 __device__ __forceinline__ void gather_conf(const Tables& t, const WalkerLane& L, int c, const S (&src)[NM],
@@ -325,12 +372,15 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
     MoSink<T, NM, false> sink{(const T*)prm.mo_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
     ao_row<T, RAD, false>(t, prm.bas_exp, prm.atom_coords, x, y, z, sink);
-    double kee, ken, d0, d1, d2, d3, d4;
-    jastrow_and_potential<false>(t, prm, L, x, y, z, kee, ken, d0, d1, d2, d3, d4);
-    double bk[NM];
-    double logabs, kin;
-    slater_combine<NM, false>(t, prm, L, mo, bk, logabs, sgn, kin);
-    return logabs + seg_sum(0.5 * kee + ken, L.wbase, t.nelec, t.nelec);
+    const double kj = (double)jastrow_value<T>(t, prm, L, x, y, z);
+    double logabs;
+    if (t.identity_conf) {
+        slater_single<NM>(t, prm, L, mo, logabs, sgn);
+    } else {
+        double bk[NM], kin;
+        slater_combine<NM, false>(t, prm, L, mo, bk, logabs, sgn, kin);
+    }
+    return logabs + seg_sum(kj, L.wbase, t.nelec, t.nelec);
 }
 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
@@ -369,25 +419,30 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
             ny = a.sw.step_size * (2.0 * u01_closed0(rnd.y) - 1.0);
             nz = a.sw.step_size * (2.0 * u01_closed0(rnd.z) - 1.0);
         } else {
-            // Box-Muller on (rnd.x, rnd.y) and (rnd.z, rnd.w)
-            const double r1 = sqrt(-2.0 * log(u01_open0(rnd.x)));
-            const double r2 = sqrt(-2.0 * log(u01_open0(rnd.z)));
-            double s1, c1, s2, c2;
-            sincospi(2.0 * u01_closed0(rnd.y), &s1, &c1);
-            sincospi(2.0 * u01_closed0(rnd.w), &s2, &c2);
-            nx = sd * r1 * c1;
-            ny = sd * r1 * s1;
-            nz = sd * r2 * c2;
+            // Box-Muller on (rnd.x, rnd.y) and (rnd.z, rnd.w), single precision:
+            // 24-bit uniforms in (0,1], radius sqrt(-2 ln u), angle 2 pi v
+            const float u1 = ((float)(rnd.x >> 8) + 1.0f) * 5.9604644775390625e-8f;
+            const float u2 = ((float)(rnd.z >> 8) + 1.0f) * 5.9604644775390625e-8f;
+            const float v1 = (float)(rnd.y >> 8) * 5.9604644775390625e-8f;
+            const float v2 = (float)(rnd.w >> 8) * 5.9604644775390625e-8f;
+            const float r1 = sqrtf(-2.0f * logf(u1));
+            const float r2 = sqrtf(-2.0f * logf(u2));
+            float s1, c1, s2, c2;
+            sincospif(2.0f * v1, &s1, &c1);
+            sincospif(2.0f * v2, &s2, &c2);
+            nx = sd * (double)(r1 * c1);
+            ny = sd * (double)(r1 * s1);
+            nz = sd * (double)(r2 * c2);
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
         double lpn = eval_logpsi<T, NM, RAD>(t, a.p, L, xn, yn, zn, sgn_n);
         if (!(lpn > lp_floor)) lpn = lp_floor;
-        // acceptance uniform: electron slot nelec of the same counter family
+        // acceptance uniform: counter slot 0x80000000 of the same walker and step;
+        // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
         const u4 ua = philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u}, k0, k1);
         const double U = u01_closed0(ua.x);
-        const double ratio = exp(fmin(2.0 * (lpn - lp), 0.0));
-        const bool acc = ratio - U >= 0.0;
+        const bool acc = 2.0 * (lpn - lp) >= (U > 0.0 ? (double)logf((float)U) : -INFINITY);
         if (acc) {
             x = xn;
             y = yn;

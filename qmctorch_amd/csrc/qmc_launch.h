@@ -23,12 +23,16 @@ inline dim3 grid_for(const Tables& t, int64_t nw) {
     return dim3((unsigned)((waves + (kBlock / 64) - 1) / (kBlock / 64)));
 }
 
-int launch_wf_f32(const KArgs& a, int mode, hipStream_t s);
-int launch_wf_f64(const KArgs& a, int mode, hipStream_t s);
-int launch_sweep_f32(const SArgs& a, hipStream_t s);
-int launch_sweep_f64(const SArgs& a, hipStream_t s);
-int launch_grad(const GArgs& a, hipStream_t s);
-int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
-int launch_eloc_grad(const EArgs& a, hipStream_t s);
+// Each family is compiled once per radial type (-DQMC_RAD=0 gto_pure, 1 sto);
+// the specialisations live in different translation units.
+template <int RAD> int launch_wf_f32(const KArgs& a, int mode, hipStream_t s);
+template <int RAD> int launch_wf_f64(const KArgs& a, int mode, hipStream_t s);
+template <int RAD> int launch_sweep_f32(const SArgs& a, hipStream_t s);
+template <int RAD> int launch_sweep_f64(const SArgs& a, hipStream_t s);
+template <int RAD> int launch_grad(const GArgs& a, hipStream_t s);
+template <int RAD> int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
+template <int RAD> int launch_eloc_grad(const EArgs& a, hipStream_t s);
+
+#define QMC_BY_RAD(fn, rad, ...) ((rad) == QMC_RADIAL_STO ? fn<QMC_RADIAL_STO>(__VA_ARGS__) : fn<QMC_RADIAL_GTO_PURE>(__VA_ARGS__))
 
 }  // namespace qmc

@@ -1,6 +1,10 @@
 // Persistent Metropolis sweep kernels, float bulk precision.
 #include "qmc_launch.h"
 
+#ifndef QMC_RAD
+#error "compile with -DQMC_RAD=0 (gto_pure) or -DQMC_RAD=1 (sto)"
+#endif
+
 namespace qmc {
 
 template <int NM, int RAD>
@@ -20,8 +24,9 @@ static int bucket(const SArgs& a, hipStream_t s) {
     }
 }
 
-int launch_sweep_f32(const SArgs& a, hipStream_t s) {
-    return a.t.radial == QMC_RADIAL_STO ? bucket<QMC_RADIAL_STO>(a, s) : bucket<QMC_RADIAL_GTO_PURE>(a, s);
+template <>
+int launch_sweep_f32<QMC_RAD>(const SArgs& a, hipStream_t s) {
+    return bucket<QMC_RAD>(a, s);
 }
 
 }  // namespace qmc

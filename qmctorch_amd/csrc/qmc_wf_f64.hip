@@ -1,6 +1,10 @@
 // psi / local-energy kernels, double bulk precision.
 #include "qmc_launch.h"
 
+#ifndef QMC_RAD
+#error "compile with -DQMC_RAD=0 (gto_pure) or -DQMC_RAD=1 (sto)"
+#endif
+
 namespace qmc {
 
 template <int NM, int RAD>
@@ -24,8 +28,9 @@ static int bucket(const KArgs& a, int mode, hipStream_t s) {
     }
 }
 
-int launch_wf_f64(const KArgs& a, int mode, hipStream_t s) {
-    return a.t.radial == QMC_RADIAL_STO ? bucket<QMC_RADIAL_STO>(a, mode, s) : bucket<QMC_RADIAL_GTO_PURE>(a, mode, s);
+template <>
+int launch_wf_f64<QMC_RAD>(const KArgs& a, int mode, hipStream_t s) {
+    return bucket<QMC_RAD>(a, mode, s);
 }
 
 }  // namespace qmc
