@@ -13,8 +13,10 @@ initialised around the nuclei.  MO coefficients are seeded random orthonormal
 Prints one JSON line (rank 0).  Besides the metric it reports the local-energy
 throughput, the local-energy error of the mixed-precision path against the
 FP64 path on the sampled positions, the roofline position of the sweep kernel
-(HIP events on the launch stream) and a CPU baseline: the oracle restatement
-of the reference algorithm on the host cores, on a bounded sample.
+(HIP events on the launch stream) and two baselines of the oracle restatement
+of the reference algorithm: on the host cores (bounded sample) and through
+torch-ROCm on the same GPU at the same walker count (the reference's own
+GPU-PyTorch path).
 """
 import argparse
 import json
@@ -52,27 +54,58 @@ def build_system(name):
             "c6h6": systems.benzene}[name]()
 
 
+def _restatement(mol, wf_dev):
+    from oracle import vmc_oracle as O
+
+    ee, en = wf_dev._jastrow_terms()
+    return O, O.OracleSlaterJastrow(mol.basis, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown,
+                                    jastrow_ee=1.0 if ee is not None else None,
+                                    jastrow_en=1.0 if en is not None else None)
+
+
 def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=150, warm=2):
-    """Oracle restatement of the reference Metropolis step on the host CPU."""
+    """Reference baselines: the oracle restatement of the reference Metropolis
+    step (host proposal RNG, per-step host<->device copies, FP64 ATen ops) timed
+    on the host CPU cores and, on the same GPU, through torch-ROCm -- the
+    reference's own GPU-PyTorch path (gpu_torch_baseline)."""
     import torch
 
-    from oracle import vmc_oracle as O
     from tests import helpers
 
+    O, orc = _restatement(mol, wf_dev)
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
-    ee, en = wf_dev._jastrow_terms()
-    orc = O.OracleSlaterJastrow(mol.basis, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown,
-                                jastrow_ee=1.0 if ee is not None else None, jastrow_en=1.0 if en is not None else None)
     pos = helpers.positions(mol, sample_walkers, seed=3)
     O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0)
     t0 = time.perf_counter()
     O.metropolis(orc.pdf, pos.clone(), sample_steps, step_size=0.2, seed=0)
     dt = time.perf_counter() - t0
-    return {"value": sample_walkers * sample_steps / dt, "unit": "walker-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": "%d walkers x %d Metropolis steps of the same system, torch CPU float64 (%.1f s)"
-                      % (sample_walkers, sample_steps, dt)}
+    cpu = {"value": sample_walkers * sample_steps / dt, "unit": "walker-steps/s", "cores": cores,
+           "kind": "port",
+           "sample": "%d walkers x %d Metropolis steps of the same system, torch CPU float64 (%.1f s)"
+                     % (sample_walkers, sample_steps, dt)}
+    return cpu
+
+
+def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=40, warm=3):
+    """The same restatement on torch-ROCm (FP64, same walker count as the
+    workload): the reference's GPU-PyTorch walker-steps/s on this GPU."""
+    import torch
+
+    from tests import helpers
+
+    O, orc = _restatement(mol, wf_dev)
+    orc.to_device("cuda")
+    pos = helpers.positions(mol, nw, seed=3).cuda()
+    O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    O.metropolis(orc.pdf, pos.clone(), sample_steps, step_size=0.2, seed=0, device="cuda")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": nw * sample_steps / dt, "unit": "walker-steps/s", "kind": "port",
+            "sample": "%d walkers x %d Metropolis steps, torch-ROCm float64 on the same GPU (%.1f s)"
+                      % (nw, sample_steps, dt)}
 
 
 def main():
@@ -222,6 +255,9 @@ def main():
         res["copy_calibration_bytes"] = {"read": src.numel() * 8, "write": dst.numel() * 8}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(mol, wf)
+        g = gpu_torch_baseline(mol, wf, nw)
+        res["reference_gpu_torch"] = g
+        res["speedup_vs_reference_gpu_torch"] = value / g["value"]
     print(json.dumps(res), flush=True)
 
 

@@ -62,6 +62,7 @@ typedef struct QmcSystemDesc {
     const int32_t* grp_prim_src; /* [sum nprim] index into bas_exp */
     const int32_t* grp_ao0;      /* [ngroup] offset into AO entries */
     const int32_t* grp_nao;      /* [ngroup] */
+    const int32_t* grp_tight;    /* [ngroup] 1: evaluated in FP64 by the mixed local energy, or NULL */
     int32_t nentry;              /* number of AO entries */
     const int32_t* gao_mu;       /* [nentry] */
     const int32_t* gao_k;        /* [nentry] */
@@ -88,8 +89,9 @@ typedef struct QmcPlan QmcPlan;
  * optimizer updates made in place are seen without re-planning:
  *   bas_exp     ao.bas_exp            [nprim_ref]   (atomic_orbitals.py:53-55)
  *   atom_coords ao.atom_coords        [natom*3]     (atomic_orbitals.py:33-37)
- *   mo_w        mo_scf * mo_modifier  [nao*nmo]     float32 (MIXED) or float64,
- *               row-major, first nmo columns (molecular_orbitals.py:79-95)
+ *   mo_w        mo_scf * mo_modifier  [nao*nmo]     row-major, first nmo columns
+ *               (molecular_orbitals.py:79-95); float32 for MIXED psi and sweeps,
+ *               float64 otherwise (FP64 runs and every local-energy evaluation)
  *   ci          fc.weight             [nci]         (slater_jastrow.py:179-191)
  *   w_ee, w_en  Pade weights          [1] or NULL   (pade_jastrow_kernel.py:28)
  */

@@ -323,7 +323,7 @@ class JastrowEE:
         return self._tri(ee_distance(pos, self.nelec, d)).view(nb, 3, -1)
 
     def _scatter(self, v, sign_col):
-        out = torch.zeros(list(v.shape[:-1]) + [self.nelec], dtype=v.dtype)
+        out = torch.zeros(list(v.shape[:-1]) + [self.nelec], dtype=v.dtype, device=v.device)
         out = out.index_add(-1, self.index_row, v)
         return out.index_add(-1, self.index_col, sign_col * v)
 
@@ -494,7 +494,7 @@ class OracleSlaterJastrow(torch.nn.Module):
             bas = self.bas_coeffs * bas if bas.ndim == 3 else self.bas_coeffs.unsqueeze(-1) * bas
         shape = list(bas.shape)
         shape[2] = self.norb
-        out = torch.zeros(shape, dtype=bas.dtype)
+        out = torch.zeros(shape, dtype=bas.dtype, device=bas.device)
         return out.index_add(2, self.index_ctr, bas)
 
     def ao(self, pos, derivative=0):
@@ -551,6 +551,21 @@ class OracleSlaterJastrow(torch.nn.Module):
     def fc(self, x):
         return x @ self.fc_weight.T
 
+    def to_device(self, device):
+        """Move every tensor of the restatement (parameters and plain tensors) to
+        ``device`` -- used to time the same op sequence on torch-ROCm."""
+        self.to(device)
+        for k, v in list(vars(self).items()):
+            if isinstance(v, torch.Tensor) and not isinstance(v, torch.nn.Parameter):
+                setattr(self, k, v.to(device))
+        self.ucfg = tuple(c.to(device) for c in self.ucfg)
+        self.iucfg = tuple(c.to(device) for c in self.iucfg)
+        for t in self.jterms:  # their weights are this module's Parameters, moved in place above
+            for k, v in list(vars(t).items()):
+                if isinstance(v, torch.Tensor) and not isinstance(v, torch.nn.Parameter):
+                    setattr(t, k, v.to(device))
+        return self
+
     # ------------------------------------------------------------ jastrow
     def jastrow(self, pos, derivative=0):
         if not self.jterms:
@@ -597,15 +612,16 @@ class OracleSlaterJastrow(torch.nn.Module):
         """-1/2 Laplacian(psi)/psi by double autograd (wf_base.py:134-182)."""
         out = self.forward(pos)
         jac = torch.autograd.grad(out, pos, torch.ones_like(out), create_graph=True)[0]
-        hess = torch.zeros(jac.shape[0], dtype=DT)
+        hess = torch.zeros(jac.shape[0], dtype=DT, device=pos.device)
         for i in range(jac.shape[1]):
-            t = torch.autograd.grad(jac[:, i], pos, torch.ones(jac.shape[0], dtype=DT), retain_graph=True)[0]
+            t = torch.autograd.grad(jac[:, i], pos, torch.ones(jac.shape[0], dtype=DT, device=pos.device),
+                                    retain_graph=True)[0]
             hess = hess + t[:, i]
         return -0.5 * hess.view(-1, 1) / out
 
     def electronic_potential(self, pos):
         """sum_{i<j} 1/r_ij with direct differences (wf_base.py:49-70)."""
-        pot = torch.zeros(pos.shape[0], dtype=DT)
+        pot = torch.zeros(pos.shape[0], dtype=DT, device=pos.device)
         for i in range(self.nelec - 1):
             pi = pos[:, 3 * i : 3 * i + 3]
             for j in range(i + 1, self.nelec):
@@ -615,7 +631,7 @@ class OracleSlaterJastrow(torch.nn.Module):
 
     def nuclear_potential(self, pos):
         """-sum Z/r_iA (wf_base.py:72-95)."""
-        p = torch.zeros(pos.shape[0], dtype=DT)
+        p = torch.zeros(pos.shape[0], dtype=DT, device=pos.device)
         for i in range(self.nelec):
             pe = pos[:, 3 * i : 3 * i + 3]
             for a in range(self.natom):
@@ -642,7 +658,7 @@ class OracleSlaterJastrow(torch.nn.Module):
         mo = self.mo(self.ao(pos))
         _, gao, _ = self.ao(pos, "all")
         dmo = self.mo(gao.transpose(2, 3)).transpose(2, 3).permute(3, 0, 1, 2)
-        eye = torch.eye(self.nelec, dtype=DT)
+        eye = torch.eye(self.nelec, dtype=DT, device=pos.device)
         dmo = (dmo.unsqueeze(2) * eye.unsqueeze(-1)).permute(2, 0, 1, 3, 4)
         dmo = dmo.reshape(-1, *dmo.shape[2:])
         gdets = self._operator_multi(mo, dmo)

@@ -34,7 +34,7 @@ class QmcSystemDesc(ctypes.Structure):
         ("radial_type", ctypes.c_int32),
         ("ngroup", ctypes.c_int32),
         ("grp_atom", c_i32p), ("grp_nprim", c_i32p), ("grp_prim0", c_i32p), ("grp_prim_src", c_i32p),
-        ("grp_ao0", c_i32p), ("grp_nao", c_i32p),
+        ("grp_ao0", c_i32p), ("grp_nao", c_i32p), ("grp_tight", c_i32p),
         ("nentry", ctypes.c_int32),
         ("gao_mu", c_i32p), ("gao_k", c_i32p), ("gao_K", c_f64p), ("gao_n", c_f64p), ("gao_src", c_i32p),
         ("atom_z", c_f64p), ("jastrow_atoms", c_f64p),

@@ -113,6 +113,7 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.grp_prim_src = p->upload(d->grp_prim_src, nprim_tot);
         t.grp_ao0 = p->upload(d->grp_ao0, d->ngroup);
         t.grp_nao = p->upload(d->grp_nao, d->ngroup);
+        t.grp_tight = p->upload(d->grp_tight, d->grp_tight ? d->ngroup : 0);
         t.gao_mu = p->upload(d->gao_mu, d->nentry);
         t.gao_k = p->upload(d->gao_k, d->nentry);
         t.gao_K = p->upload(d->gao_K, (size_t)d->nentry * QMC_MAXP);

@@ -23,6 +23,7 @@ struct Tables {
     const int* grp_prim_src;
     const int* grp_ao0;
     const int* grp_nao;
+    const int* grp_tight;
     const int* gao_mu;
     const int* gao_k;
     const double* gao_K;
@@ -163,28 +164,48 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
     }
 }
 
-template <typename T, int RAD, bool DERIV, typename Sink>
+// TIGHT: groups flagged in grp_tight are evaluated in FP64 whatever T is (the
+// mixed-precision local energy: tight core primitives cancel in lap AO near the
+// nucleus, which FP32 cannot resolve).
+template <typename T, int RAD, bool DERIV, bool TIGHT = false, typename Sink>
 __device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict__ bas_exp,
                                        const double* __restrict__ atom_xyz, double ex, double ey, double ez,
                                        Sink&& sink) {
     int cur_atom = -1;
+    double dxd = 0, dyd = 0, dzd = 0, r2d = 0, rd = 0, ird = 0;
     T dx = 0, dy = 0, dz = 0, r2 = 0, r = 0, ir = 0;
     for (int g = 0; g < t.ngroup; ++g) {
         const int at = t.grp_atom[g];
         if (at != cur_atom) {
             cur_atom = at;
-            dx = (T)(ex - atom_xyz[3 * at + 0]);
-            dy = (T)(ey - atom_xyz[3 * at + 1]);
-            dz = (T)(ez - atom_xyz[3 * at + 2]);
+            dxd = ex - atom_xyz[3 * at + 0];
+            dyd = ey - atom_xyz[3 * at + 1];
+            dzd = ez - atom_xyz[3 * at + 2];
+            dx = (T)dxd;
+            dy = (T)dyd;
+            dz = (T)dzd;
             r2 = dx * dx + dy * dy + dz * dz;
             if (RAD == QMC_RADIAL_STO) {
                 r = fsqrt(r2);
                 ir = T(1) / r;
             }
+            if (TIGHT) {
+                r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+                if (RAD == QMC_RADIAL_STO) {
+                    rd = sqrt(r2d);
+                    ird = 1.0 / rd;
+                }
+            }
         }
+        const bool tight = TIGHT && t.grp_tight != nullptr && t.grp_tight[g] != 0;
         switch (t.grp_nprim[g]) {
-#define QMC_AO_CASE(K) \
-    case K: ao_group<T, RAD, DERIV, K>(t, g, bas_exp, dx, dy, dz, r2, r, ir, sink); break;
+#define QMC_AO_CASE(K)                                                                               \
+    case K:                                                                                          \
+        if (tight)                                                                                   \
+            ao_group<double, RAD, DERIV, K>(t, g, bas_exp, dxd, dyd, dzd, r2d, rd, ird, sink);       \
+        else                                                                                         \
+            ao_group<T, RAD, DERIV, K>(t, g, bas_exp, dx, dy, dz, r2, r, ir, sink);                  \
+        break;
             QMC_AO_CASE(1) QMC_AO_CASE(2) QMC_AO_CASE(3) QMC_AO_CASE(4) QMC_AO_CASE(5) QMC_AO_CASE(6)
             QMC_AO_CASE(7) QMC_AO_CASE(8) QMC_AO_CASE(9) QMC_AO_CASE(10) QMC_AO_CASE(11) QMC_AO_CASE(12)
 #undef QMC_AO_CASE

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "qmc_device.h"
 
@@ -45,28 +46,31 @@ __device__ __forceinline__ WalkerLane lane_layout(const Tables& t, int64_t nw) {
     return L;
 }
 
-// MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95)
-template <typename T, int NM, bool DERIV>
+// MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95).
+// WT is the storage type of W, AT the accumulator type; AO values may arrive in
+// either precision (FP64 for tight exponent groups in the mixed local energy).
+template <typename WT, int NM, bool DERIV, typename AT = WT>
 struct MoSink {
-    const T* __restrict__ W;
+    const WT* __restrict__ W;
     int nmo;
-    T* mo;
-    T* gx;
-    T* gy;
-    T* gz;
-    T* lp;
-    __device__ __forceinline__ void operator()(int mu, T ao, T ax, T ay, T az, T al) const {
-        const T* __restrict__ wr = W + (size_t)mu * nmo;
+    AT* mo;
+    AT* gx;
+    AT* gy;
+    AT* gz;
+    AT* lp;
+    template <typename V>
+    __device__ __forceinline__ void operator()(int mu, V ao, V ax, V ay, V az, V al) const {
+        const WT* __restrict__ wr = W + (size_t)mu * nmo;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             if (m < nmo) {
-                const T w = wr[m];
-                mo[m] += ao * w;
+                const AT w = (AT)wr[m];
+                mo[m] += (AT)ao * w;
                 if (DERIV) {
-                    gx[m] += ax * w;
-                    gy[m] += ay * w;
-                    gz[m] += az * w;
-                    lp[m] += al * w;
+                    gx[m] += (AT)ax * w;
+                    gy[m] += (AT)ay * w;
+                    gz[m] += (AT)az * w;
+                    lp[m] += (AT)al * w;
                 }
             }
         }
@@ -321,11 +325,14 @@ __global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
     const double x = p[0], y = p[1], z = p[2];
 
-    T mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
+    // local energy: MO rows accumulated in FP64 from an FP64 W; AO groups with
+    // tight exponents in FP64 even when the bulk type T is float
+    using AT = typename std::conditional<ELOC, double, T>::type;
+    AT mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
 #pragma unroll
-    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = T(0);
-    MoSink<T, NM, ELOC> sink{(const T*)a.p.mo_w, t.nmo, mo, gx, gy, gz, lp};
-    ao_row<T, RAD, ELOC>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = AT(0);
+    MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.mo_w, t.nmo, mo, gx, gy, gz, lp};
+    ao_row<T, RAD, ELOC, ELOC>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
 
     double kee, ken, jgx, jgy, jgz, jlap, vpot;
     jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vpot);

@@ -74,6 +74,10 @@ def atomic_orbital_norm(basis) -> np.ndarray:
 class SystemTables:
     """Numpy tables of one wavefunction; owns the arrays QmcSystemDesc points to."""
 
+    # groups whose largest exponent reaches this value are evaluated in FP64 by the
+    # mixed-precision local energy (core primitives: O 1s 11720, Li 1s 642, ...)
+    TIGHT_EXPONENT = 50.0
+
     def __init__(self, basis, atom_coords, atomic_number, nup, ndown, configs, nmo_used,
                  use_ee, use_en, norm_cst, bas_exp_values):
         self.basis = basis
@@ -121,7 +125,7 @@ class SystemTables:
             groups[key].append(mu)
         order.sort(key=lambda k: k[0])  # groups of one atom are contiguous
 
-        g_atom, g_nprim, g_prim0, g_src, g_ao0, g_nao = [], [], [], [], [], []
+        g_atom, g_nprim, g_prim0, g_src, g_ao0, g_nao, g_tight = [], [], [], [], [], [], []
         e_mu, e_k, e_K, e_n, e_src = [], [], [], [], []
         P = N.QMC_MAXP
         for key in order:
@@ -136,6 +140,7 @@ class SystemTables:
                 g_src.extend(rep[c0:c1].tolist())
                 g_ao0.append(len(e_mu))
                 g_nao.append(len(aos))
+                g_tight.append(1 if max(exps) >= self.TIGHT_EXPONENT else 0)
                 for mu in aos:
                     ent = entries[mu][c0:c1]
                     k0 = entries[mu][0]
@@ -157,6 +162,7 @@ class SystemTables:
         self.grp_prim_src = np.asarray(g_src, dtype=np.int32)
         self.grp_ao0 = np.asarray(g_ao0, dtype=np.int32)
         self.grp_nao = np.asarray(g_nao, dtype=np.int32)
+        self.grp_tight = np.asarray(g_tight, dtype=np.int32)
         self.gao_mu = np.asarray(e_mu, dtype=np.int32)
         self.gao_k = np.asarray(e_k, dtype=np.int32)
         self.gao_K = np.ascontiguousarray(np.stack(e_K)).astype(np.float64)
@@ -219,6 +225,7 @@ class SystemTables:
         d.ngroup = len(self.grp_atom)
         d.grp_atom, d.grp_nprim, d.grp_prim0 = ip(self.grp_atom), ip(self.grp_nprim), ip(self.grp_prim0)
         d.grp_prim_src, d.grp_ao0, d.grp_nao = ip(self.grp_prim_src), ip(self.grp_ao0), ip(self.grp_nao)
+        d.grp_tight = ip(self.grp_tight)
         d.nentry = len(self.gao_mu)
         d.gao_mu, d.gao_k, d.gao_K, d.gao_n = ip(self.gao_mu), ip(self.gao_k), fp(self.gao_K), fp(self.gao_n)
         d.gao_src = ip(self.gao_src)

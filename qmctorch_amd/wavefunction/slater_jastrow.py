@@ -176,7 +176,8 @@ class SlaterJastrow(WaveFunction):
         x = self._check_pos(pos)
         nw = x.shape[0]
         plan = self.device_plan()
-        prm, keep = self._params(precision)
+        # the local-energy kernel reads W in FP64 in both precisions
+        prm, keep = self._params(precision if mode == N.QMC_EVAL_PSI else "fp64")
         out = {k: torch.empty(nw, dtype=torch.float64, device=x.device)
                for k in (("psi", "logabs", "sign") if mode == N.QMC_EVAL_PSI
                          else ("psi", "logabs", "sign", "kinetic", "eloc", "epot"))}
