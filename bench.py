@@ -63,7 +63,7 @@ def _restatement(mol, wf_dev):
                                     jastrow_en=1.0 if en is not None else None)
 
 
-def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=150, warm=2):
+def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=300, warm=2):
     """Reference baselines: the oracle restatement of the reference Metropolis
     step (host proposal RNG, per-step host<->device copies, FP64 ATen ops) timed
     on the host CPU cores and, on the same GPU, through torch-ROCm -- the
@@ -87,7 +87,7 @@ def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=150, warm=2):
     return cpu
 
 
-def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=40, warm=3):
+def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=200, warm=3):
     """The same restatement on torch-ROCm (FP64, same walker count as the
     workload): the reference's GPU-PyTorch walker-steps/s on this GPU."""
     import torch
@@ -108,6 +108,72 @@ def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=40, warm=3):
                       % (nw, sample_steps, dt)}
 
 
+def batched_uneven(nsys=64, steps=100, seed=0, precision="mixed"):
+    """64 systems drawn from {H2, LiH, H2O, N2, C6H6}, walkers log-uniform
+    256-8192: sweeps of all systems in one batched call per chunk of steps
+    (one launch per register bucket) vs the same sweeps one system per launch."""
+    import torch
+
+    from qmctorch_amd.serving import BatchRunner, SystemContext, WalkerSet, catalog, sweep_each
+
+    draw = catalog.uneven_batch(nsys, seed=seed)
+    ctx = {n: SystemContext(catalog.wavefunction(n, precision), n) for n in sorted({d[0] for d in draw})}
+
+    def walkers():
+        return [WalkerSet(ctx[n], catalog.initial_positions(ctx[n].wf.mol, nw, 1000 + i), 1000 + i)
+                for i, (n, nw) in enumerate(draw)]
+
+    runner = BatchRunner(max(nsys, 1))
+    total = sum(nw for _, nw in draw) * steps
+    chunk = 50
+    res = {}
+    for mode in ("batched", "per_system"):
+        ws = walkers()
+        warm = [(w, 5) for w in ws]
+        runner.sweep(warm) if mode == "batched" else sweep_each(warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            n = min(chunk, steps - done)
+            jobs = [(w, n) for w in ws]
+            runner.sweep(jobs) if mode == "batched" else sweep_each(jobs)
+            done += n
+        torch.cuda.synchronize()
+        res[mode] = total / (time.perf_counter() - t0)
+    mix = {}
+    for n, _ in draw:
+        mix[n] = mix.get(n, 0) + 1
+    return {"systems": nsys, "mix": mix, "walkers": sum(nw for _, nw in draw), "steps": steps,
+            "walker_steps_per_s_batched": res["batched"], "walker_steps_per_s_per_system": res["per_system"],
+            "speedup": res["batched"] / res["per_system"]}
+
+
+def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0):
+    """Poisson request streams (config-4 molecule set, walkers 128-16384,
+    1-20 sweeps + one E_L batch): continuous batching vs the plain
+    per-request path on the same GPU."""
+    from qmctorch_amd.serving import (SystemContext, VMCServer, catalog, latency_summary, poisson_requests,
+                                      serve_per_request)
+
+    ctx = {n: SystemContext(catalog.wavefunction(n, precision), n) for n in catalog.SYSTEM_NAMES}
+    server = VMCServer(precision=precision, steps_per_tick=steps_per_tick, contexts=ctx)
+    server.run(poisson_requests(1000.0, 10, seed=99), realtime=False)  # warm every kernel
+    out = []
+    for rate in rates:
+        reqs = poisson_requests(rate, nreq, seed=seed)
+        t0 = time.perf_counter()
+        res = server.run(reqs, realtime=True)
+        row = {"rate_req_per_s": rate, "batched": latency_summary(res, time.perf_counter() - t0),
+               "peak_requests_in_flight": server.peak_active}
+        if plain:
+            t0 = time.perf_counter()
+            res_p = serve_per_request(reqs, ctx, realtime=True)
+            row["per_request"] = latency_summary(res_p, time.perf_counter() - t0)
+        out.append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +184,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--steps-per-launch", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-system batched-launch leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-loop leg")
+    ap.add_argument("--serving-rates", default="500", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -246,6 +316,14 @@ def main():
             "flops_per_walker_step": F + 4 * T,
         },
     }
+    if not args.no_batched:
+        res["batched_uneven"] = batched_uneven(precision=args.precision)
+    if not args.no_serving:
+        rows = serving([float(r) for r in args.serving_rates.split(",")], args.serving_requests,
+                       precision=args.precision)
+        res["serving"] = rows
+        res["p50_ms"] = rows[0]["batched"]["p50_ms"]
+        res["p99_ms"] = rows[0]["batched"]["p99_ms"]
     if args.copy_calibration:
         src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).uniform_()
         dst = torch.empty_like(src)

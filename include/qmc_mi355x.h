@@ -206,6 +206,50 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
                        double* ao_rows, double* b_rows, double* ga_rows, double* gb_rows, double* dci,
                        double* djas, double* drift, qmc_stream_t stream);
 
+/*
+ * Batched launches: many systems (plans), each with its own parameters,
+ * walkers, walker count and step count, in one launch per register bucket
+ * (occupied orbitals per spin <= 2, 4, 8, 16, 24) and radial type.  Waves of
+ * the costliest systems are dispatched first.  A walker's arithmetic does not
+ * depend on the other jobs of a batch, so every result equals that of a batch
+ * holding its job alone, bit for bit.  No counterpart in the reference, whose
+ * driver handles one wavefunction per Solver (solver_base.py:316-387); the
+ * callers are request loops that run uneven systems side by side.
+ *
+ * A QmcBatch owns a ring of job tables (pinned host staging + device copy) for
+ * up to max_jobs jobs per call; a call reuses a slot only after the launches
+ * that last read it have completed.  One QmcBatch serves one host thread.
+ */
+typedef struct QmcBatch QmcBatch;
+
+typedef struct QmcSweepJob {
+    const QmcPlan* plan;
+    QmcParams params;
+    QmcWalkers walkers;
+    int64_t nw;
+    QmcSweepDesc sweep;
+} QmcSweepJob;
+
+typedef struct QmcEvalJob {
+    const QmcPlan* plan;
+    QmcParams params;
+    const double* pos;
+    int64_t nw;
+    QmcWfOut out;
+} QmcEvalJob;
+
+int qmc_batch_create(int32_t max_jobs, QmcBatch** batch);
+int qmc_batch_destroy(QmcBatch* batch);
+
+/* qmc_metropolis_sweep for every job of the list (jobs with nw == 0 or
+ * nstep <= 0 are skipped). */
+int qmc_metropolis_sweep_batch(QmcBatch* batch, int32_t precision, const QmcSweepJob* jobs, int32_t njobs,
+                               qmc_stream_t stream);
+
+/* qmc_wf_eval for every job of the list. */
+int qmc_wf_eval_batch(QmcBatch* batch, int32_t precision, int32_t mode, const QmcEvalJob* jobs, int32_t njobs,
+                      qmc_stream_t stream);
+
 /* Philox4x32-10 of the sweep kernel applied to n (counter[4], key[2]) tuples
  * (device pointers: in [n*6] uint32, out [n*4] uint32).  No counterpart in the
  * reference (whose proposals come from torch's host generator); exposed so the

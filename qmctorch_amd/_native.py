@@ -76,6 +76,20 @@ class QmcSweepDesc(ctypes.Structure):
     ]
 
 
+class QmcSweepJob(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p), ("params", QmcParams), ("walkers", QmcWalkers), ("nw", ctypes.c_int64),
+        ("sweep", QmcSweepDesc),
+    ]
+
+
+class QmcEvalJob(ctypes.Structure):
+    _fields_ = [
+        ("plan", ctypes.c_void_p), ("params", QmcParams), ("pos", ctypes.c_void_p), ("nw", ctypes.c_int64),
+        ("out", QmcWfOut),
+    ]
+
+
 EXPORTS = {
     "qmc_plan_create": ([ctypes.POINTER(QmcSystemDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "qmc_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
@@ -91,6 +105,12 @@ EXPORTS = {
                              ctypes.c_int),
     "qmc_eloc_grad_rows": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64]
                            + [ctypes.c_void_p] * 8 + [ctypes.c_void_p], ctypes.c_int),
+    "qmc_batch_create": ([ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "qmc_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "qmc_metropolis_sweep_batch": ([ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(QmcSweepJob), ctypes.c_int32,
+                                    ctypes.c_void_p], ctypes.c_int),
+    "qmc_wf_eval_batch": ([ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(QmcEvalJob),
+                           ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
     "qmc_philox4x32_10": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "qmc_last_error": ([], ctypes.c_char_p),
     "qmc_version": ([], ctypes.c_char_p),

@@ -18,7 +18,8 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libqmcmi355x.so")
 # (source, object tag, extra defines): kernel families are compiled once per radial type
-_FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip", "qmc_grad.hip"]
+_FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip", "qmc_grad.hip",
+             "qmc_batch_sweep_f32.hip", "qmc_batch_sweep_f64.hip", "qmc_batch_wf_f32.hip", "qmc_batch_wf_f64.hip"]
 UNITS = [("qmc_abi.hip", "qmc_abi", [])] + [
     (f, f.replace(".hip", "_r%d" % r), ["-DQMC_RAD=%d" % r]) for f in _FAMILIES for r in (0, 1)
 ]

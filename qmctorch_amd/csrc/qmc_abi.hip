@@ -107,29 +107,29 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.identity_conf = ident ? 1 : 0;
         int nprim_tot = 0;
         for (int g = 0; g < d->ngroup; ++g) nprim_tot = std::max(nprim_tot, d->grp_prim0[g] + d->grp_nprim[g]);
-        t.grp_atom = p->upload(d->grp_atom, d->ngroup);
-        t.grp_nprim = p->upload(d->grp_nprim, d->ngroup);
-        t.grp_prim0 = p->upload(d->grp_prim0, d->ngroup);
-        t.grp_prim_src = p->upload(d->grp_prim_src, nprim_tot);
-        t.grp_ao0 = p->upload(d->grp_ao0, d->ngroup);
-        t.grp_nao = p->upload(d->grp_nao, d->ngroup);
-        t.grp_tight = p->upload(d->grp_tight, d->grp_tight ? d->ngroup : 0);
-        t.gao_mu = p->upload(d->gao_mu, d->nentry);
-        t.gao_k = p->upload(d->gao_k, d->nentry);
-        t.gao_K = p->upload(d->gao_K, (size_t)d->nentry * QMC_MAXP);
+        t.grp_atom = (decltype(t.grp_atom))p->upload(d->grp_atom, d->ngroup);
+        t.grp_nprim = (decltype(t.grp_nprim))p->upload(d->grp_nprim, d->ngroup);
+        t.grp_prim0 = (decltype(t.grp_prim0))p->upload(d->grp_prim0, d->ngroup);
+        t.grp_prim_src = (decltype(t.grp_prim_src))p->upload(d->grp_prim_src, nprim_tot);
+        t.grp_ao0 = (decltype(t.grp_ao0))p->upload(d->grp_ao0, d->ngroup);
+        t.grp_nao = (decltype(t.grp_nao))p->upload(d->grp_nao, d->ngroup);
+        t.grp_tight = (decltype(t.grp_tight))p->upload(d->grp_tight, d->grp_tight ? d->ngroup : 0);
+        t.gao_mu = (decltype(t.gao_mu))p->upload(d->gao_mu, d->nentry);
+        t.gao_k = (decltype(t.gao_k))p->upload(d->gao_k, d->nentry);
+        t.gao_K = (decltype(t.gao_K))p->upload(d->gao_K, (size_t)d->nentry * QMC_MAXP);
         {
             std::vector<float> kf((size_t)d->nentry * QMC_MAXP);
             for (size_t i = 0; i < kf.size(); ++i) kf[i] = (float)d->gao_K[i];
-            t.gao_Kf = p->upload(kf.data(), kf.size());
+            t.gao_Kf = (decltype(t.gao_Kf))p->upload(kf.data(), kf.size());
         }
-        t.gao_n = p->upload(d->gao_n, (size_t)d->nentry * QMC_MAXP);
-        t.gao_src = p->upload(d->gao_src, (size_t)d->nentry * QMC_MAXP);
-        t.atom_z = p->upload(d->atom_z, d->natom);
-        t.jas_atoms = p->upload(d->jastrow_atoms, (size_t)d->natom * 3);
-        t.conf_up = p->upload(d->conf_up, (size_t)d->nconf_up * d->nup);
-        t.conf_dn = p->upload(d->conf_dn, (size_t)d->nconf_dn * (d->ndown > 0 ? d->ndown : 1));
-        t.ci_up = p->upload(d->ci_up, d->nci);
-        t.ci_dn = p->upload(d->ci_dn, d->nci);
+        t.gao_n = (decltype(t.gao_n))p->upload(d->gao_n, (size_t)d->nentry * QMC_MAXP);
+        t.gao_src = (decltype(t.gao_src))p->upload(d->gao_src, (size_t)d->nentry * QMC_MAXP);
+        t.atom_z = (decltype(t.atom_z))p->upload(d->atom_z, d->natom);
+        t.jas_atoms = (decltype(t.jas_atoms))p->upload(d->jastrow_atoms, (size_t)d->natom * 3);
+        t.conf_up = (decltype(t.conf_up))p->upload(d->conf_up, (size_t)d->nconf_up * d->nup);
+        t.conf_dn = (decltype(t.conf_dn))p->upload(d->conf_dn, (size_t)d->nconf_dn * (d->ndown > 0 ? d->ndown : 1));
+        t.ci_up = (decltype(t.ci_up))p->upload(d->ci_up, d->nci);
+        t.ci_dn = (decltype(t.ci_dn))p->upload(d->ci_dn, d->nci);
         if (!t.conf_dn) t.conf_dn = t.conf_up;
     } catch (const std::exception& ex) {
         delete p;
@@ -291,6 +291,226 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
     fill_zero(b_rows, n, s);
     if (QMC_BY_RAD(launch_eloc_grad, a.t.radial, a, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+}  // extern "C"
+
+// ========================================================================= batch
+struct QmcBatch {
+    static constexpr int kSlots = 8;
+    int max_jobs = 0;
+    size_t slot_bytes = 0;
+    char* host = nullptr;  // pinned staging, kSlots * slot_bytes
+    char* dev = nullptr;   // device job tables, kSlots * slot_bytes
+    hipEvent_t done[kSlots] = {};
+    bool pending[kSlots] = {};
+    int next = 0;
+    ~QmcBatch() {
+        for (int i = 0; i < kSlots; ++i) {
+            if (pending[i]) (void)hipEventSynchronize(done[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+        }
+        if (host) (void)hipHostFree(host);
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+namespace {
+
+// Relative per-wave cost of a system (lane work: AO groups x MO columns,
+// pair terms, elimination), used to dispatch the costliest waves first.
+double wave_cost(const Tables& t) {
+    return (double)t.nentry * (t.nmo + 8) + 12.0 * t.ngroup + 6.0 * t.nelec * (t.nelec + t.natom) +
+           (double)t.nsig_max * t.nsig_max * t.nconf_max;
+}
+
+// Takes the next ring slot, waiting (rarely) for the launches that last read it.
+int take_slot(QmcBatch* b, int& slot) {
+    slot = b->next;
+    b->next = (b->next + 1) % QmcBatch::kSlots;
+    if (b->pending[slot]) {
+        QMC_HIP(hipEventSynchronize(b->done[slot]));
+        b->pending[slot] = false;
+    }
+    return QMC_OK;
+}
+
+struct Group {
+    int rad, bucket;
+    std::vector<int> idx;
+};
+
+// Groups the jobs by (radial type, register bucket), costliest systems first
+// inside a group.  Returns an error for jobs without a bucket.
+template <typename JobT>
+int group_jobs(const JobT* jobs, int njobs, std::vector<Group>& groups) {
+    for (int j = 0; j < njobs; ++j) {
+        const Tables& t = jobs[j].plan->t;
+        const int bk = nm_bucket(t);
+        if (bk < 0) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+        Group* g = nullptr;
+        for (auto& x : groups)
+            if (x.rad == t.radial && x.bucket == bk) g = &x;
+        if (!g) {
+            groups.push_back(Group{t.radial, bk, {}});
+            g = &groups.back();
+        }
+        g->idx.push_back(j);
+    }
+    for (auto& g : groups)
+        std::stable_sort(g.idx.begin(), g.idx.end(),
+                         [&](int a, int b) { return wave_cost(jobs[a].plan->t) > wave_cost(jobs[b].plan->t); });
+    return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qmc_batch_create(int32_t max_jobs, QmcBatch** out) {
+    if (!out || max_jobs <= 0) return fail(QMC_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    QmcBatch* b = new QmcBatch();
+    b->max_jobs = max_jobs;
+    const size_t per = std::max(sizeof(SJob), sizeof(WJob));
+    b->slot_bytes = ((per * (size_t)max_jobs + 255) / 256) * 256;
+    const size_t total = b->slot_bytes * QmcBatch::kSlots;
+    if (hipHostMalloc((void**)&b->host, total, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&b->dev, total) != hipSuccess) {
+        delete b;
+        return fail(QMC_ERR_HIP, "batch table allocation failed");
+    }
+    for (int i = 0; i < QmcBatch::kSlots; ++i) {
+        if (hipEventCreateWithFlags(&b->done[i], hipEventDisableTiming) != hipSuccess) {
+            delete b;
+            return fail(QMC_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    *out = b;
+    return QMC_OK;
+}
+
+int qmc_batch_destroy(QmcBatch* b) {
+    delete b;
+    return QMC_OK;
+}
+
+int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob* jobs, int32_t njobs,
+                               qmc_stream_t stream) {
+    if (!b || (njobs > 0 && !jobs)) return fail(QMC_ERR_ARG, "null batch or jobs");
+    if (njobs < 0 || njobs > b->max_jobs) return fail(QMC_ERR_ARG, "job count outside [0, max_jobs]");
+    std::vector<QmcSweepJob> live;
+    for (int j = 0; j < njobs; ++j) {
+        const QmcSweepJob& x = jobs[j];
+        int rc = check_common(x.plan, &x.params, x.nw);
+        if (rc) return rc;
+        if (!x.walkers.pos || !x.walkers.logabs || !x.walkers.sign) return fail(QMC_ERR_ARG, "null walker state");
+        if (x.sweep.keep && x.sweep.keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
+        if (x.nw > 0 && x.sweep.nstep > 0) live.push_back(x);
+    }
+    if (live.empty()) return QMC_OK;
+    std::vector<Group> groups;
+    int rc = group_jobs(live.data(), (int)live.size(), groups);
+    if (rc) return rc;
+    int slot;
+    rc = take_slot(b, slot);
+    if (rc) return rc;
+    SJob* host = (SJob*)(b->host + slot * b->slot_bytes);
+    SJob* dev = (SJob*)(b->dev + slot * b->slot_bytes);
+    struct Launch {
+        int rad, bucket, first, n, nwaves;
+    };
+    std::vector<Launch> launches;
+    int k = 0;
+    for (const auto& g : groups) {
+        int64_t waves = 0;
+        const int first = k;
+        for (int j : g.idx) {
+            const QmcSweepJob& x = live[j];
+            SJob& e = host[k++];
+            e.a.t = x.plan->t;
+            e.a.p = x.params;
+            e.a.wk = x.walkers;
+            e.a.sw = x.sweep;
+            e.a.nw = x.nw;
+            e.wave0 = waves;
+            const int G = 64 / x.plan->t.nelec;
+            waves += (x.nw + G - 1) / G;
+        }
+        if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves});
+    }
+    hipStream_t s = (hipStream_t)stream;
+    QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(SJob), hipMemcpyHostToDevice, s));
+    for (const auto& L : launches) {
+        const int r = precision == QMC_PREC_FP64
+                          ? QMC_BY_RAD(launch_sweep_batch_f64, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, s)
+                          : QMC_BY_RAD(launch_sweep_batch_f32, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, s);
+        if (r) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    }
+    QMC_HIP(hipGetLastError());
+    QMC_HIP(hipEventRecord(b->done[slot], s));
+    b->pending[slot] = true;
+    return QMC_OK;
+}
+
+int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEvalJob* jobs, int32_t njobs,
+                      qmc_stream_t stream) {
+    if (!b || (njobs > 0 && !jobs)) return fail(QMC_ERR_ARG, "null batch or jobs");
+    if (njobs < 0 || njobs > b->max_jobs) return fail(QMC_ERR_ARG, "job count outside [0, max_jobs]");
+    std::vector<QmcEvalJob> live;
+    for (int j = 0; j < njobs; ++j) {
+        const QmcEvalJob& x = jobs[j];
+        int rc = check_common(x.plan, &x.params, x.nw);
+        if (rc) return rc;
+        if (!x.pos) return fail(QMC_ERR_ARG, "null pos");
+        if (x.nw > 0) live.push_back(x);
+    }
+    if (live.empty()) return QMC_OK;
+    std::vector<Group> groups;
+    int rc = group_jobs(live.data(), (int)live.size(), groups);
+    if (rc) return rc;
+    int slot;
+    rc = take_slot(b, slot);
+    if (rc) return rc;
+    WJob* host = (WJob*)(b->host + slot * b->slot_bytes);
+    WJob* dev = (WJob*)(b->dev + slot * b->slot_bytes);
+    struct Launch {
+        int rad, bucket, first, n, nwaves;
+    };
+    std::vector<Launch> launches;
+    int k = 0;
+    for (const auto& g : groups) {
+        int64_t waves = 0;
+        const int first = k;
+        for (int j : g.idx) {
+            const QmcEvalJob& x = live[j];
+            WJob& e = host[k++];
+            e.a.t = x.plan->t;
+            e.a.p = x.params;
+            e.a.pos = x.pos;
+            e.a.nw = x.nw;
+            e.a.out = x.out;
+            e.wave0 = waves;
+            const int G = 64 / x.plan->t.nelec;
+            waves += (x.nw + G - 1) / G;
+        }
+        if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves});
+    }
+    hipStream_t s = (hipStream_t)stream;
+    QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(WJob), hipMemcpyHostToDevice, s));
+    for (const auto& L : launches) {
+        const int r =
+            precision == QMC_PREC_FP64
+                ? QMC_BY_RAD(launch_wf_batch_f64, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, mode, s)
+                : QMC_BY_RAD(launch_wf_batch_f32, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, mode, s);
+        if (r) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    }
+    QMC_HIP(hipGetLastError());
+    QMC_HIP(hipEventRecord(b->done[slot], s));
+    b->pending[slot] = true;
     return QMC_OK;
 }
 

@@ -1,0 +1,28 @@
+// Batched persistent Metropolis sweeps (many systems per launch), double bulk precision.
+#include "qmc_launch.h"
+
+#ifndef QMC_RAD
+#error "compile with -DQMC_RAD=0 (gto_pure) or -DQMC_RAD=1 (sto)"
+#endif
+
+namespace qmc {
+
+template <int NM, int RAD>
+static void go(const SJob* jobs, int njobs, int nwaves, hipStream_t s) {
+    hipLaunchKernelGGL((sweep_batch_kernel<double, NM, RAD>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs, njobs,
+                       nwaves);
+}
+
+template <>
+int launch_sweep_batch_f64<QMC_RAD>(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s) {
+    switch (bucket) {
+        case 2: go<2, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+        case 4: go<4, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+        case 8: go<8, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+        case 16: go<16, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+        case 24: go<24, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+        default: return -1;
+    }
+}
+
+}  // namespace qmc

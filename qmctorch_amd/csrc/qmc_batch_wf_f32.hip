@@ -1,0 +1,32 @@
+// Batched psi / local-energy evaluation (many systems per launch), float bulk precision.
+#include "qmc_launch.h"
+
+#ifndef QMC_RAD
+#error "compile with -DQMC_RAD=0 (gto_pure) or -DQMC_RAD=1 (sto)"
+#endif
+
+namespace qmc {
+
+template <int NM, int RAD>
+static void go(const WJob* jobs, int njobs, int nwaves, int mode, hipStream_t s) {
+    if (mode == QMC_EVAL_ELOC)
+        hipLaunchKernelGGL((wf_batch_kernel<float, NM, RAD, true>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs,
+                           njobs, nwaves);
+    else
+        hipLaunchKernelGGL((wf_batch_kernel<float, NM, RAD, false>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs,
+                           njobs, nwaves);
+}
+
+template <>
+int launch_wf_batch_f32<QMC_RAD>(const WJob* jobs, int njobs, int nwaves, int bucket, int mode, hipStream_t s) {
+    switch (bucket) {
+        case 2: go<2, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+        case 4: go<4, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+        case 8: go<8, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+        case 16: go<16, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+        case 24: go<24, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+        default: return -1;
+    }
+}
+
+}  // namespace qmc

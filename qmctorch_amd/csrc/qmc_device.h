@@ -12,30 +12,45 @@
 
 namespace qmc {
 
+// The tables and parameters of a launch are never written while its kernels
+// run: device code reads them through the constant address space, so that
+// wave-uniform reads become scalar loads whether the pointer arrived as a
+// kernel argument or from a job table in device memory.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define QMC_CONST __attribute__((address_space(4)))
+#else
+#define QMC_CONST
+#endif
+
+template <typename X>
+__device__ __forceinline__ X ldc(const X* p, size_t i) {
+    return ((const __attribute__((address_space(4))) X*)p)[i];
+}
+
 struct Tables {
     int nelec, nup, ndown, natom, nao, nprim_ref, radial;
     int ngroup, nentry, nmo, nci, nconf_up, nconf_dn, nconf_max, nsig_max;
     int use_ee, use_en, identity_conf;
     double vnn;
-    const int* grp_atom;
-    const int* grp_nprim;
-    const int* grp_prim0;
-    const int* grp_prim_src;
-    const int* grp_ao0;
-    const int* grp_nao;
-    const int* grp_tight;
-    const int* gao_mu;
-    const int* gao_k;
-    const double* gao_K;
-    const float* gao_Kf;
-    const double* gao_n;
-    const int* gao_src;
-    const double* atom_z;
-    const double* jas_atoms;
-    const int* conf_up;
-    const int* conf_dn;
-    const int* ci_up;
-    const int* ci_dn;
+    const QMC_CONST int* grp_atom;
+    const QMC_CONST int* grp_nprim;
+    const QMC_CONST int* grp_prim0;
+    const QMC_CONST int* grp_prim_src;
+    const QMC_CONST int* grp_ao0;
+    const QMC_CONST int* grp_nao;
+    const QMC_CONST int* grp_tight;
+    const QMC_CONST int* gao_mu;
+    const QMC_CONST int* gao_k;
+    const QMC_CONST double* gao_K;
+    const QMC_CONST float* gao_Kf;
+    const QMC_CONST double* gao_n;
+    const QMC_CONST int* gao_src;
+    const QMC_CONST double* atom_z;
+    const QMC_CONST double* jas_atoms;
+    const QMC_CONST int* conf_up;
+    const QMC_CONST int* conf_dn;
+    const QMC_CONST int* ci_up;
+    const QMC_CONST int* ci_dn;
 };
 
 // ------------------------------------------------------------------ math
@@ -88,13 +103,13 @@ __device__ __forceinline__ void cart_poly(T x, T y, T z, int kx, int ky, int kz,
 // are linear in the AO).  The primitive count of a group selects a fully
 // unrolled instance (no predicated slots).
 template <typename T>
-__device__ __forceinline__ const T* coeff_table(const Tables& t);
+__device__ __forceinline__ const QMC_CONST T* coeff_table(const Tables& t);
 template <>
-__device__ __forceinline__ const float* coeff_table<float>(const Tables& t) {
+__device__ __forceinline__ const QMC_CONST float* coeff_table<float>(const Tables& t) {
     return t.gao_Kf;
 }
 template <>
-__device__ __forceinline__ const double* coeff_table<double>(const Tables& t) {
+__device__ __forceinline__ const QMC_CONST double* coeff_table<double>(const Tables& t) {
     return t.gao_K;
 }
 
@@ -105,18 +120,18 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
     T E[NP], A[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const T a = (T)bas_exp[t.grp_prim_src[p0 + p]];
+        const T a = (T)ldc(bas_exp, t.grp_prim_src[p0 + p]);
         A[p] = a;
         E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
     }
-    const T* __restrict__ Ktab = coeff_table<T>(t);
+    const QMC_CONST T* Ktab = coeff_table<T>(t);
     const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
     for (int i = i0; i < i1; ++i) {
         const int kk = t.gao_k[i];
         const int kx = kk & 15, ky = (kk >> 4) & 15, kz = (kk >> 8) & 15;
         T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
         cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
-        const T* Kp = Ktab + (size_t)i * QMC_MAXP;
+        const QMC_CONST T* Kp = Ktab + (size_t)i * QMC_MAXP;
         T s0 = 0, s1 = 0, s2 = 0;
         if (RAD == QMC_RADIAL_GTO_PURE) {
 #pragma unroll
@@ -138,7 +153,7 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
                 sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
             }
         } else {
-            const double* Np = t.gao_n + (size_t)i * QMC_MAXP;
+            const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
             const T L = T(kx + ky + kz);
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
@@ -178,9 +193,9 @@ __device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict
         const int at = t.grp_atom[g];
         if (at != cur_atom) {
             cur_atom = at;
-            dxd = ex - atom_xyz[3 * at + 0];
-            dyd = ey - atom_xyz[3 * at + 1];
-            dzd = ez - atom_xyz[3 * at + 2];
+            dxd = ex - ldc(atom_xyz, 3 * at + 0);
+            dyd = ey - ldc(atom_xyz, 3 * at + 1);
+            dzd = ez - ldc(atom_xyz, 3 * at + 2);
             dx = (T)dxd;
             dy = (T)dyd;
             dz = (T)dzd;

@@ -24,10 +24,11 @@ struct WalkerLane {
     uint64_t gmask;
 };
 
-__device__ __forceinline__ WalkerLane lane_layout(const Tables& t, int64_t nw) {
+// wave: index of this wavefront among the waves serving the nw walkers
+__device__ __forceinline__ WalkerLane lane_layout_w(const Tables& t, int64_t nw, int wave) {
     WalkerLane L;
     L.lane = threadIdx.x & 63;
-    L.wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    L.wave = wave;
     const int Ne = t.nelec;
     L.G = 64 / Ne;
     L.wl = L.lane / Ne;
@@ -46,6 +47,31 @@ __device__ __forceinline__ WalkerLane lane_layout(const Tables& t, int64_t nw) {
     return L;
 }
 
+__device__ __forceinline__ int global_wave() {
+    return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6));
+}
+
+__device__ __forceinline__ WalkerLane lane_layout(const Tables& t, int64_t nw) {
+    return lane_layout_w(t, nw, global_wave());
+}
+
+// Index of the job owning global wave `wave` in a batched launch: the last j
+// with wave0[j] <= wave (wave0 ascending, wave0[0] = 0).  The job table is read
+// before the kernel stores anything, so these wave-uniform loads and the copy
+// of the job's arguments go through the scalar cache like kernel arguments.
+template <typename J>
+__device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs, int wave) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].wave0 <= wave)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
 // MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95).
 // WT is the storage type of W, AT the accumulator type; AO values may arrive in
 // either precision (FP64 for tight exponent groups in the mixed local energy).
@@ -60,7 +86,7 @@ struct MoSink {
     AT* lp;
     template <typename V>
     __device__ __forceinline__ void operator()(int mu, V ao, V ax, V ay, V az, V al) const {
-        const WT* __restrict__ wr = W + (size_t)mu * nmo;
+        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)mu * nmo;
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             if (m < nmo) {
@@ -88,8 +114,8 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
                                                       double& vpot) {
     kee = ken = gx = gy = gz = lap = vpot = 0.0;
     const int Ne = t.nelec;
-    const double wee = (t.use_ee && prm.w_ee) ? prm.w_ee[0] : 0.0;
-    const double wen = (t.use_en && prm.w_en) ? prm.w_en[0] : 0.0;
+    const double wee = (t.use_ee && prm.w_ee) ? ldc(prm.w_ee, 0) : 0.0;
+    const double wen = (t.use_en && prm.w_en) ? ldc(prm.w_en, 0) : 0.0;
     for (int j = 0; j < Ne; ++j) {
         const double xj = __shfl(x, L.wbase + j);
         const double yj = __shfl(y, L.wbase + j);
@@ -134,8 +160,8 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
     }
     if (DERIV) {
         for (int A = 0; A < t.natom; ++A) {
-            const double dx = x - prm.atom_coords[3 * A], dy = y - prm.atom_coords[3 * A + 1],
-                         dz = z - prm.atom_coords[3 * A + 2];
+            const double dx = x - ldc(prm.atom_coords, 3 * A), dy = y - ldc(prm.atom_coords, 3 * A + 1),
+                         dz = z - ldc(prm.atom_coords, 3 * A + 2);
             vpot -= t.atom_z[A] / sqrt(dx * dx + dy * dy + dz * dz);
         }
     }
@@ -148,7 +174,7 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
                                            double y, double z) {
     T k = T(0);
     if (t.use_ee) {
-        const T wee = (T)prm.w_ee[0];
+        const T wee = (T)ldc(prm.w_ee, 0);
         const T xs = (T)x, ys = (T)y, zs = (T)z;
         for (int j = 0; j < t.nelec; ++j) {
             const T xj = __shfl(xs, L.wbase + j), yj = __shfl(ys, L.wbase + j), zj = __shfl(zs, L.wbase + j);
@@ -159,7 +185,7 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
         }
     }
     if (t.use_en) {
-        const T wen = (T)prm.w_en[0];
+        const T wen = (T)ldc(prm.w_en, 0);
         for (int A = 0; A < t.natom; ++A) {
             const T dx = (T)(x - t.jas_atoms[3 * A]), dy = (T)(y - t.jas_atoms[3 * A + 1]),
                     dz = (T)(z - t.jas_atoms[3 * A + 2]);
@@ -183,7 +209,7 @@ __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& 
     const int dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
     const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
     const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
-    const double c0 = prm.ci[0];
+    const double c0 = ldc(prm.ci, 0);
     logabs = lu + (t.ndown > 0 ? ldn : 0.0) + log(fabs(c0));
     sgn = su * (t.ndown > 0 ? sdn : 1.0) * (c0 < 0.0 ? -1.0 : 1.0);
 }
@@ -197,7 +223,7 @@ __device__ __forceinline__ void gather_conf(const Tables& t, const WalkerLane& L
         for (int j = 0; j < NM; ++j) dst[j] = (double)src[j];
         return;
     }
-    const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+    const QMC_CONST int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
     const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
     const bool has = c < nconf;
 #pragma unroll
@@ -299,7 +325,7 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
                 if (KIN) tr += to[c];
             }
         }
-        const double dn = prm.ci[n] * sg * exp(l - lmax);
+        const double dn = ldc(prm.ci, n) * sg * exp(l - lmax);
         s += dn;
         if (KIN) st += tr * dn;
     }
@@ -316,12 +342,10 @@ struct KArgs {
     QmcWfOut out;
 };
 
-// psi / kinetic / local energy of nw walkers.
+// psi / kinetic / local energy of the walkers of one wavefront.
 template <typename T, int NM, int RAD, bool ELOC>
-__global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
+__device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
-    const WalkerLane L = lane_layout(t, a.nw);
-    if ((int64_t)L.wave * L.G >= a.nw) return;  // whole wavefront beyond the batch
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
     const double x = p[0], y = p[1], z = p[2];
 
@@ -363,6 +387,30 @@ __global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
     }
 }
 
+template <typename T, int NM, int RAD, bool ELOC>
+__global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;  // whole wavefront beyond the batch
+    wf_walkers<T, NM, RAD, ELOC>(a, L);
+}
+
+// Many systems in one launch: job j owns global waves [wave0[j], wave0[j+1]).
+struct WJob {
+    KArgs a;
+    int64_t wave0;
+};
+
+template <typename T, int NM, int RAD, bool ELOC>
+__global__ __launch_bounds__(kBlock) void wf_batch_kernel(const WJob* __restrict__ jobs, int njobs, int nwaves) {
+    const int wave = global_wave();
+    if (wave >= nwaves) return;
+    const int j = find_job(jobs, njobs, wave);
+    const KArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    wf_walkers<T, NM, RAD, ELOC>(a, L);
+}
+
 struct SArgs {
     Tables t;
     QmcParams p;
@@ -393,10 +441,8 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
 // state in registers (sampler/metropolis.py:134-165, 256-298).
 template <typename T, int NM, int RAD>
-__global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
+__device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
-    const WalkerLane L = lane_layout(t, a.nw);
-    if ((int64_t)L.wave * L.G >= a.nw) return;
     const int Ne = t.nelec;
     double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
     double x = pp[0], y = pp[1], z = pp[2];
@@ -479,6 +525,32 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
     }
 }
 
+template <typename T, int NM, int RAD>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD>(a, L);
+}
+
+// Sweeps of many systems (each with its own walkers, step count and seed) in
+// one launch; job j owns global waves [wave0[j], wave0[j+1]).  The arithmetic
+// of a walker does not depend on the other jobs of the launch.
+struct SJob {
+    SArgs a;
+    int64_t wave0;
+};
+
+template <typename T, int NM, int RAD>
+__global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restrict__ jobs, int njobs, int nwaves) {
+    const int wave = global_wave();
+    if (wave >= nwaves) return;
+    const int j = find_job(jobs, njobs, wave);
+    const SArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD>(a, L);
+}
+
 // AO values and derivatives in the reference layout (atomic_orbitals.py:578-609).
 struct AArgs {
     Tables t;
@@ -556,7 +628,7 @@ __device__ __forceinline__ void conf_dets(const Tables& t, const WalkerLane& L, 
 #pragma unroll
             for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
                 if (c2 == c) w = om[c2];
-            const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+            const QMC_CONST int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
             const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
             // the lane that pivoted column k holds row k of A^-1 in bb; this
             // lane needs (A^-1)[k][myrow] for every k
@@ -593,15 +665,15 @@ __device__ __forceinline__ void exp_adjoint(const Tables& t, const QmcParams& pr
     const double* W = (const double*)prm.mo_w;
     for (int g = 0; g < t.ngroup; ++g) {
         const int at = t.grp_atom[g];
-        const double dx = ex - prm.atom_coords[3 * at], dy = ey - prm.atom_coords[3 * at + 1],
-                     dz = ez - prm.atom_coords[3 * at + 2];
+        const double dx = ex - ldc(prm.atom_coords, 3 * at), dy = ey - ldc(prm.atom_coords, 3 * at + 1),
+                     dz = ez - ldc(prm.atom_coords, 3 * at + 2);
         const double r2 = dx * dx + dy * dy + dz * dz, r = sqrt(r2);
         const int np = t.grp_nprim[g], p0 = t.grp_prim0[g];
         const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
         for (int i = i0; i < i1; ++i) {
             const int mu = t.gao_mu[i];
             double G = 0.0;
-            const double* wr = W + (size_t)mu * t.nmo;
+            const QMC_CONST double* wr = (const QMC_CONST double*)W + (size_t)mu * t.nmo;
 #pragma unroll
             for (int m = 0; m < NM; ++m)
                 if (m < t.nmo) G += gam[m] * wr[m];
@@ -609,7 +681,7 @@ __device__ __forceinline__ void exp_adjoint(const Tables& t, const QmcParams& pr
             double Y, d0, d1, d2, d3;
             cart_poly<double, false>(dx, dy, dz, kk & 15, (kk >> 4) & 15, (kk >> 8) & 15, Y, d0, d1, d2, d3);
             for (int p = 0; p < np; ++p) {
-                const double a = prm.bas_exp[t.grp_prim_src[p0 + p]];
+                const double a = ldc(prm.bas_exp, t.grp_prim_src[p0 + p]);
                 const double K = t.gao_K[(size_t)i * QMC_MAXP + p];
                 double term;
                 if (RAD == QMC_RADIAL_GTO_PURE) {
@@ -643,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
         const double* W = (const double*)a.p.mo_w;
         auto sink = [&](int mu, double val, double, double, double, double) {
             if (v) aor[mu] += val;
-            const double* wr = W + (size_t)mu * t.nmo;
+            const QMC_CONST double* wr = (const QMC_CONST double*)W + (size_t)mu * t.nmo;
 #pragma unroll
             for (int m = 0; m < NM; ++m)
                 if (m < t.nmo) mo[m] += val * wr[m];
@@ -697,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
     for (int n = 0; n < t.nci; ++n) {
         double l, sg;
         term(n, l, sg);
-        S += a.p.ci[n] * sg * exp(l - lmax);
+        S += ldc(a.p.ci, n) * sg * exp(l - lmax);
     }
     for (int n = 0; n < t.nci; ++n) {
         double l, sg;
@@ -706,7 +778,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
         const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
 #pragma unroll
         for (int c = 0; c < QMC_MAXCONF; ++c)
-            if (c == mine) om[c] += a.p.ci[n] * dn_s;
+            if (c == mine) om[c] += ldc(a.p.ci, n) * dn_s;
         if (v && L.e == 0) a.dci[(size_t)L.w * t.nci + n] = dn_s;
     }
     conf_dets<NM, true>(t, L, mo, ld, sd, om, gam);
@@ -719,8 +791,8 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
     if (a.dexp_rows) exp_adjoint<NM, RAD>(t, a.p, x, y, z, gam, v ? a.dexp_rows + row * t.nprim_ref : nullptr);
     // Jastrow weights: d/dw [w0 r/(1+w r)] = -w0 r^2/(1+w r)^2
     double dee = 0.0, den_ = 0.0;
-    const double wee = (t.use_ee && a.p.w_ee) ? a.p.w_ee[0] : 0.0;
-    const double wen = (t.use_en && a.p.w_en) ? a.p.w_en[0] : 0.0;
+    const double wee = (t.use_ee && a.p.w_ee) ? ldc(a.p.w_ee, 0) : 0.0;
+    const double wen = (t.use_en && a.p.w_en) ? ldc(a.p.w_en, 0) : 0.0;
     for (int j = 0; j < t.nelec; ++j) {
         const double xj = __shfl(x, L.wbase + j), yj = __shfl(y, L.wbase + j), zj = __shfl(z, L.wbase + j);
         if (t.use_ee && j != L.e) {
@@ -779,8 +851,8 @@ __device__ __forceinline__ void jastrow_w_derivs(const Tables& t, const QmcParam
                                                  double& dle, double (&dgn)[3], double& dln) {
     for (int d = 0; d < 3; ++d) g[d] = dge[d] = dgn[d] = 0.0;
     l = dle = dln = 0.0;
-    const double wee = (t.use_ee && prm.w_ee) ? prm.w_ee[0] : 0.0;
-    const double wen = (t.use_en && prm.w_en) ? prm.w_en[0] : 0.0;
+    const double wee = (t.use_ee && prm.w_ee) ? ldc(prm.w_ee, 0) : 0.0;
+    const double wen = (t.use_en && prm.w_en) ? ldc(prm.w_en, 0) : 0.0;
     for (int j = 0; j < t.nelec; ++j) {
         const double xj = __shfl(x, L.wbase + j), yj = __shfl(y, L.wbase + j), zj = __shfl(z, L.wbase + j);
         if (t.use_ee && j != L.e) {
@@ -893,7 +965,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
         auto sink = [&](int mu, double ao, double ax, double ay, double az, double al) {
             if (v && aor) aor[mu] += ao;
             if (v && bor) bor[mu] += -0.5 * (al + 2.0 * (g[0] * ax + g[1] * ay + g[2] * az) + l2 * ao);
-            const double* wr = W + (size_t)mu * t.nmo;
+            const QMC_CONST double* wr = (const QMC_CONST double*)W + (size_t)mu * t.nmo;
 #pragma unroll
             for (int m = 0; m < NM; ++m) {
                 if (m < t.nmo) {
@@ -980,7 +1052,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
     for (int n = 0; n < t.nci; ++n) {
         double lg, sg, tr;
         term(n, lg, sg, tr);
-        const double dn = a.p.ci[n] * sg * exp(lg - lmax);
+        const double dn = ldc(a.p.ci, n) * sg * exp(lg - lmax);
         S += dn;
         ST += dn * tr;
     }
@@ -992,7 +1064,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
         double lg, sg, tr;
         term(n, lg, sg, tr);
         const double dns = sg * exp(lg - lmax) / S;  // D_n / S
-        const double wn = a.p.ci[n] * dns;
+        const double wn = ldc(a.p.ci, n) * dns;
         const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
 #pragma unroll
         for (int c = 0; c < QMC_MAXCONF; ++c)
@@ -1028,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
                 rc = rho[c2];
                 oc = om[c2];
             }
-        const int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+        const QMC_CONST int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
         const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
         if (c < nconf) {
 #pragma unroll

@@ -32,6 +32,15 @@ template <int RAD> int launch_sweep_f64(const SArgs& a, hipStream_t s);
 template <int RAD> int launch_grad(const GArgs& a, hipStream_t s);
 template <int RAD> int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
 template <int RAD> int launch_eloc_grad(const EArgs& a, hipStream_t s);
+// batched launches: jobs (device) sorted into one register bucket, nwaves in total
+template <int RAD> int launch_sweep_batch_f32(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s);
+template <int RAD> int launch_sweep_batch_f64(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s);
+template <int RAD>
+int launch_wf_batch_f32(const WJob* jobs, int njobs, int nwaves, int bucket, int mode, hipStream_t s);
+template <int RAD>
+int launch_wf_batch_f64(const WJob* jobs, int njobs, int nwaves, int bucket, int mode, hipStream_t s);
+
+inline dim3 grid_for_waves(int nwaves) { return dim3((unsigned)((nwaves + (kBlock / 64) - 1) / (kBlock / 64))); }
 
 #define QMC_BY_RAD(fn, rad, ...) ((rad) == QMC_RADIAL_STO ? fn<QMC_RADIAL_STO>(__VA_ARGS__) : fn<QMC_RADIAL_GTO_PURE>(__VA_ARGS__))
 

@@ -66,3 +66,47 @@ def test_vnn_h2o():
     z = [8, 1, 1]
     ref = sum(z[a] * z[b] / np.linalg.norm(c[a] - c[b]) for a in range(3) for b in range(a + 1, 3))
     assert abs(t.vnn - ref) < 1e-12
+
+
+def test_poisson_requests_are_seeded_and_within_ranges():
+    from qmctorch_amd.serving import poisson_requests
+
+    a = poisson_requests(500.0, 4000, seed=7)
+    b = poisson_requests(500.0, 4000, seed=7)
+    assert [(r.system, r.nwalkers, r.nsweeps, r.arrival) for r in a] == \
+        [(r.system, r.nwalkers, r.nsweeps, r.arrival) for r in b]
+    t = np.array([r.arrival for r in a])
+    assert np.all(np.diff(t) > 0)
+    assert abs(len(a) / t[-1] - 500.0) < 30.0  # mean rate of the Poisson process
+    w = np.array([r.nwalkers for r in a])
+    k = np.array([r.nsweeps for r in a])
+    assert w.min() >= 128 and w.max() <= 16384 and k.min() == 1 and k.max() == 20
+    # log-uniform walker counts: the median of exp(U(ln a, ln b)) is sqrt(a b)
+    assert abs(np.log(np.median(w)) - 0.5 * np.log(128 * 16384)) < 0.1
+    assert {r.system for r in a} == {"h2", "lih", "h2o", "n2", "c6h6"}
+
+
+def test_uneven_batch_draw():
+    from qmctorch_amd.serving import catalog
+
+    jobs = catalog.uneven_batch(64, seed=0)
+    assert len(jobs) == 64 and jobs == catalog.uneven_batch(64, seed=0)
+    assert all(256 <= n <= 8192 for _, n in jobs)
+    assert {s for s, _ in jobs} <= set(catalog.SYSTEM_NAMES)
+
+
+def test_latency_summary():
+    from qmctorch_amd.serving.server import VMCResult, latency_summary
+
+    res = [VMCResult(i, "h2", 10, 2, 0.0, 0.0, 0.0, 0.5, np.zeros(10), 0.0, 0.0, (i + 1) * 1e-3)
+           for i in range(100)]
+    s = latency_summary(res, 2.0)
+    assert abs(s["p50_ms"] - 50.5) < 1e-9 and abs(s["p99_ms"] - 99.01) < 1e-9
+    assert s["walker_steps_per_s"] == 100 * 20 / 2.0
+
+
+def test_serving_catalog_molecules():
+    from qmctorch_amd.serving import catalog
+
+    counts = {n: (catalog.molecule(n).nelec, catalog.molecule(n).basis.nao) for n in catalog.SYSTEM_NAMES}
+    assert counts == {"h2": (2, 2), "lih": (4, 11), "h2o": (10, 25), "n2": (14, 30), "c6h6": (42, 120)}

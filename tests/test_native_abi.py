@@ -47,6 +47,8 @@ def test_ctypes_struct_layout_matches_header(tmp_path):
         "QmcWfOut": N.QmcWfOut,
         "QmcWalkers": N.QmcWalkers,
         "QmcSweepDesc": N.QmcSweepDesc,
+        "QmcSweepJob": N.QmcSweepJob,
+        "QmcEvalJob": N.QmcEvalJob,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "qmc_mi355x.h"', "int main(void) {"]
     for sname, cls in structs.items():
@@ -94,3 +96,15 @@ KAT = [  # Random123 known-answer vectors for philox4x32-10
 @pytest.mark.parametrize("ctr,key,ref", KAT)
 def test_philox_host_known_answers(ctr, key, ref):
     assert tuple(philox_host(ctr, key)) == ref
+
+
+def test_batch_entry_points_reject_bad_arguments_without_gpu_calls():
+    if not os.path.isfile(N.LIBPATH):
+        pytest.skip("libqmcmi355x.so not built")
+    lib = N.load()
+    h = ctypes.c_void_p()
+    assert lib.qmc_batch_create(0, ctypes.byref(h)) != 0 and not h.value
+    assert "batch" in lib.qmc_last_error().decode()
+    assert lib.qmc_metropolis_sweep_batch(None, N.QMC_PREC_MIXED, None, 1, None) != 0
+    assert lib.qmc_wf_eval_batch(None, N.QMC_PREC_MIXED, N.QMC_EVAL_ELOC, None, 1, None) != 0
+    assert "null" in lib.qmc_last_error().decode()
