@@ -108,6 +108,37 @@ def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=200, warm=3):
                       % (nw, sample_steps, dt)}
 
 
+def sweep_precision_effect(mol, wf, nw, steps=2000, seed=77):
+    """Effect of the FP32-bulk sweep on the sampled energy: the same chains
+    (same start, seed and walker ids) run by the FP32-bulk and the FP64 sweep;
+    a walker's trajectory differs only after an acceptance decision flips.
+    Reports the fraction of walkers whose final positions differ and the
+    difference of the FP64 energy means per electron."""
+    import torch
+
+    from qmctorch_amd.sampler import Metropolis
+    from qmctorch_amd.serving import catalog
+
+    start = catalog.initial_positions(mol, nw, seed=seed)
+    out = {}
+    for prec in ("mixed", "fp64"):
+        old = wf.precision
+        wf.precision = prec
+        try:
+            s = Metropolis(nwalkers=nw, nstep=steps, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
+                           move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=seed)
+            out[prec] = s(wf.pdf, pos=start, with_tqdm=False).detach()
+        finally:
+            wf.precision = old
+    diff = (out["mixed"] != out["fp64"]).any(dim=1)
+    e32 = wf.local_energy_parts(out["mixed"], "fp64")["eloc"]
+    e64 = wf.local_energy_parts(out["fp64"], "fp64")["eloc"]
+    err = float(torch.sqrt(e64.var() / nw)) / mol.nelec
+    return {"walkers": nw, "steps": steps, "fraction_of_walkers_diverged": float(diff.double().mean()),
+            "energy_difference_per_electron": float((e32.mean() - e64.mean()).abs()) / mol.nelec,
+            "statistical_error_per_electron": err}
+
+
 def batched_uneven(nsys=64, steps=100, seed=0, precision="mixed"):
     """64 systems drawn from {H2, LiH, H2O, N2, C6H6}, walkers log-uniform
     256-8192: sweeps of all systems in one batched call per chunk of steps
@@ -174,6 +205,92 @@ def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0
     return out
 
 
+def config_legs(precision="mixed"):
+    """The other benchmark configurations at their stated sizes (one short run each)."""
+    import numpy as np
+    import torch
+
+    from qmctorch_amd.sampler import Metropolis
+    from qmctorch_amd.serving import catalog
+    from qmctorch_amd.solver import Solver
+    from tests import helpers, systems
+
+    out = {}
+    # H2/STO-3G FP64, e-e Pade, 1000 walkers x 2000 steps, then E_L of the final positions
+    mol = systems.h2()
+    wf, _ = helpers.make_pair(mol, w_ee=1.0, perturb=0.0, precision="fp64")
+    pos0 = catalog.initial_positions(mol, 1000, seed=0)
+    sampler = Metropolis(nwalkers=1000, nstep=2000, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
+                         move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = sampler(wf.pdf, pos=pos0, with_tqdm=False).detach()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e = wf.local_energy(x).reshape(-1)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["h2_sto3g_fp64"] = {"walkers": 1000, "steps": 2000, "walker_steps_per_s": 1000 * 2000 / (t1 - t0),
+                            "local_energy_s": t2 - t1, "energy": float(e.mean()),
+                            "error": float(e.std() / np.sqrt(e.numel())), "acceptance": sampler.last_acceptance}
+
+    # LiH/6-31G CAS(2,2), CI [1, N(0, 0.1) x 3], e-e + e-n Pade: mixed vs FP64 local energy on 10^5
+    # Gaussian positions (sigma 1 bohr around the nuclei, seed 1) and 10^5 positions of an FP64 chain
+    mol = systems.lih()
+    rng = np.random.default_rng(0)
+    ci = [1.0] + list(0.1 * rng.standard_normal(3))
+    wf64, _ = helpers.make_pair(mol, configs="cas(2,2)", w_ee=1.0, w_en=1.0, ci=ci, perturb=0.0, precision="fp64")
+    wfmx, _ = helpers.make_pair(mol, configs="cas(2,2)", w_ee=1.0, w_en=1.0, ci=ci, perturb=0.0, precision="mixed")
+    gauss = catalog.initial_positions(mol, 100000, seed=1)
+    chain = Metropolis(nwalkers=10000, nstep=510, ntherm=500, step_size=0.2, nelec=mol.nelec,
+                       init=mol.domain("atomic"), move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=1)
+    cpos = chain(wf64.pdf, pos=catalog.initial_positions(mol, 10000, seed=2), with_tqdm=False).detach()
+    errs = {}
+    for name, p in (("gaussian", gauss), ("fp64_chain", cpos)):
+        e64 = wf64.local_energy_parts(p, "fp64")["eloc"]
+        emx = wfmx.local_energy_parts(p, "mixed")["eloc"]
+        d = (emx - e64).abs() / mol.nelec
+        ok = torch.isfinite(d)
+        errs[name] = {"positions": int(p.shape[0]), "max": float(d[ok].max()), "mean": float(d[ok].mean()),
+                      "median": float(d[ok].median()),
+                      "bias_of_mean": float((emx[ok].mean() - e64[ok].mean()).abs() / mol.nelec)}
+    nw = 16384
+    st = catalog.initial_positions(mol, nw, seed=3)
+    sm = Metropolis(nwalkers=nw, nstep=500, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
+                    move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=3)
+    sm(wfmx.pdf, pos=st, with_tqdm=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sm(wfmx.pdf, pos=st, with_tqdm=False)
+    torch.cuda.synchronize()
+    out["lih_631g_cas22_mixed"] = {"walkers": nw, "walker_steps_per_s": nw * 500 / (time.perf_counter() - t0),
+                                   "eloc_error_per_electron_vs_fp64": errs}
+
+    # H2O/cc-pVDZ, 8192 walkers, energy minimisation (manual gradient through the adjoint kernel,
+    # freeze ao and ci, Adam lr 1e-3, 25 resampling steps per epoch)
+    mol = systems.h2o()
+    wf, _ = helpers.make_pair(mol, w_ee=1.0, w_en=1.0, perturb=0.0, precision=precision)
+    samp = Metropolis(nwalkers=8192, nstep=100, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
+                      move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=4)
+    samp.walkers.initialize(pos=catalog.initial_positions(mol, 8192, seed=4))
+    opt = torch.optim.Adam(wf.parameters(), lr=1e-3)
+    solver = Solver(wf=wf, sampler=samp, optimizer=opt)
+    solver.configure(freeze=["ao", "ci"], track=["local_energy"])
+    solver.prepare_optimization(None, None)  # initial sampling: 100 steps from the seeded positions
+    solver.run_epochs(1, verbose=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nep = 10
+    solver.run_epochs(nep, verbose=False)
+    torch.cuda.synchronize()
+    ep = (time.perf_counter() - t0) / nep
+    obs = solver.observable
+    en = [float(v) for v in obs.energy] if hasattr(obs, "energy") else []
+    out["h2o_ccpvdz_optimisation"] = {"walkers": 8192, "epochs": nep, "s_per_epoch": ep,
+                                      "energy_first_last": en[:1] + en[-1:]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +301,7 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--steps-per-launch", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the H2 / LiH / H2O-optimisation legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-system batched-launch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-loop leg")
     ap.add_argument("--serving-rates", default="500", help="comma-separated Poisson rates (req/s)")
@@ -265,21 +383,31 @@ def main():
     flops_launch = (F + 4 * T) * nw * steps_per_launch
     achieved = flops_launch / launch_s / 1e12
 
-    # local energy throughput and mixed-vs-fp64 error on the sampled positions
+    # local-energy throughput on the sampled positions: the FP64 kernel the
+    # wavefunction uses for E_L, and the FP32-bulk kernel with its error
     pos = x.clone()
-    wf.local_energy_parts(pos, args.precision)
-    torch.cuda.synchronize()
-    reps = 5
-    t1 = time.perf_counter()
-    for _ in range(reps):
-        parts = wf.local_energy_parts(pos, args.precision)
-    torch.cuda.synchronize()
-    el_rate = reps * nw / (time.perf_counter() - t1)
+
+    def el_rate_of(prec_name):
+        wf.local_energy_parts(pos, prec_name)
+        torch.cuda.synchronize()
+        reps = 5
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            wf.local_energy_parts(pos, prec_name)
+        torch.cuda.synchronize()
+        return reps * nw / (time.perf_counter() - t1)
+
+    el_rate = el_rate_of(wf.eloc_precision)
+    el_rate_fp32 = el_rate_of("mixed")
     e64 = wf.local_energy_parts(pos, "fp64")["eloc"]
     emx = wf.local_energy_parts(pos, "mixed")["eloc"]
     d = ((emx - e64).abs() / mol.nelec)
-    err = {"max": float(d.max()), "median": float(d.median()), "mean": float(d.mean()),
-           "bias_of_mean": float((emx.mean() - e64.mean()).abs() / mol.nelec)}
+    err32 = {"max": float(d.max()), "median": float(d.median()), "mean": float(d.mean()),
+             "bias_of_mean": float((emx.mean() - e64.mean()).abs() / mol.nelec)}
+    e_used = wf.local_energy_parts(pos)["eloc"]
+    du = (e_used - e64).abs() / mol.nelec
+    err = {"max": float(du.max()), "mean": float(du.mean()), "eloc_precision": wf.eloc_precision}
+    sweep_vs64 = sweep_precision_effect(mol, wf, nw) if args.precision == "mixed" else None
 
     res = {
         "metric": METRIC,
@@ -304,6 +432,8 @@ def main():
         "acceptance": acc,
         "local_energy_per_s": el_rate,
         "eloc_error_per_electron_vs_fp64": err,
+        "local_energy_fp32_bulk": {"per_s": el_rate_fp32, "error_per_electron_vs_fp64": err32},
+        "sweep_fp32_bulk_vs_fp64_chain": sweep_vs64,
         "roofline": {
             "bound": "mfma",
             "achieved": achieved,
@@ -316,6 +446,8 @@ def main():
             "flops_per_walker_step": F + 4 * T,
         },
     }
+    if not args.no_configs:
+        res["configs"] = config_legs(args.precision)
     if not args.no_batched:
         res["batched_uneven"] = batched_uneven(precision=args.precision)
     if not args.no_serving:

@@ -255,12 +255,17 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
         if (k < nmax) {
             const bool colk = k < n;
             const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+            // all candidates of the group are fetched first (independent
+            // shuffles, one wait), then scanned in row order
+            double c[NM];
+#pragma unroll
+            for (int s = 0; s < NM; ++s) c[s] = __shfl(cand, gbase + (s < n ? s : 0));
             double best = -1.0;
             int br = 0;
-            for (int s = 0; s < nmax; ++s) {
-                const double c = __shfl(cand, gbase + (s < n ? s : 0));
-                if (s < n && c > best) {
-                    best = c;
+#pragma unroll
+            for (int s = 0; s < NM; ++s) {
+                if (s < n && c[s] > best) {
+                    best = c[s];
                     br = s;
                 }
             }
@@ -279,21 +284,19 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
             }
             const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
             const double f = (colk && !is_piv && (WITH_B || !done)) ? a[k] * ip : 0.0;
+            // pivot row broadcast: columns past nmax hold zeros and stay zero,
+            // so every column is updated without a per-column branch
+            double pa[NM];
 #pragma unroll
-            for (int j = k + 1; j < NM; ++j) {
-                if (j < nmax) {
-                    const double pj = __shfl(a[j], plane);
-                    a[j] = is_piv ? pj * ip : a[j] - f * pj;
-                }
-            }
+            for (int j = k + 1; j < NM; ++j) pa[j] = __shfl(a[j], plane);
+#pragma unroll
+            for (int j = k + 1; j < NM; ++j) a[j] = is_piv ? pa[j] * ip : a[j] - f * pa[j];
             if (WITH_B) {
+                double pb[NM];
 #pragma unroll
-                for (int j = 0; j < NM; ++j) {
-                    if (j < nmax) {
-                        const double pj = __shfl(b[j], plane);
-                        b[j] = is_piv ? pj * ip : b[j] - f * pj;
-                    }
-                }
+                for (int j = 0; j < NM; ++j) pb[j] = __shfl(b[j], plane);
+#pragma unroll
+                for (int j = 0; j < NM; ++j) b[j] = is_piv ? pb[j] * ip : b[j] - f * pb[j];
             }
             if (is_piv) {
                 done = true;
@@ -312,11 +315,16 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
 }
 
 // Sum of v over the lanes [base, base+cnt) (result in every lane; cnt uniform bound).
+// Terms are fetched eight at a time (independent shuffles) and added in lane order.
 __device__ __forceinline__ double seg_sum(double v, int base, int cnt, int cnt_max) {
     double s = 0.0;
-    for (int t = 0; t < cnt_max; ++t) {
-        const double x = __shfl(v, base + (t < cnt ? t : 0));
-        if (t < cnt) s += x;
+    for (int t0 = 0; t0 < cnt_max; t0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = __shfl(v, base + (t0 + u < cnt ? t0 + u : 0));
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t0 + u < cnt) s += x[u];
     }
     return s;
 }

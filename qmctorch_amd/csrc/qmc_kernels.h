@@ -176,12 +176,26 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
     if (t.use_ee) {
         const T wee = (T)ldc(prm.w_ee, 0);
         const T xs = (T)x, ys = (T)y, zs = (T)z;
-        for (int j = 0; j < t.nelec; ++j) {
-            const T xj = __shfl(xs, L.wbase + j), yj = __shfl(ys, L.wbase + j), zj = __shfl(zs, L.wbase + j);
-            const T dx = xs - xj, dy = ys - yj, dz = zs - zj;
-            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
-            const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
-            k += (j != L.e) ? w0 * r / (T(1) + wee * r) : T(0);
+        // partner coordinates fetched four electrons at a time (independent shuffles)
+        for (int j0 = 0; j0 < t.nelec; j0 += 4) {
+            T xj[4], yj[4], zj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jl = L.wbase + (j0 + u < t.nelec ? j0 + u : 0);
+                xj[u] = __shfl(xs, jl);
+                yj[u] = __shfl(ys, jl);
+                zj[u] = __shfl(zs, jl);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j < t.nelec) {
+                    const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
+                    const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+                    const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
+                    k += (j != L.e) ? w0 * r / (T(1) + wee * r) : T(0);
+                }
+            }
         }
     }
     if (t.use_en) {

@@ -24,6 +24,7 @@ class SystemContext:
         self.name = name or getattr(wf.mol, "name", "system")
         self.nelec = wf.nelec
         self.precision = wf.precision
+        self.eloc_precision = wf.eloc_precision
         self.plan = wf.device_plan()
         self.prm_sweep, self._keep_sweep = wf._params(wf.precision)
         self.prm_eval, self._keep_eval = wf._params("fp64")  # the local-energy kernel reads W in FP64
@@ -76,8 +77,8 @@ class BatchRunner:
             pass
 
     @staticmethod
-    def _prec(ctxs) -> int:
-        precs = {c.precision for c in ctxs}
+    def _prec(ctxs, attr="precision") -> int:
+        precs = {getattr(c, attr) for c in ctxs}
         if len(precs) != 1:
             raise ValueError("one batch runs one precision (got %s)" % sorted(precs))
         return N.QMC_PREC_MIXED if precs.pop() == "mixed" else N.QMC_PREC_FP64
@@ -115,7 +116,7 @@ class BatchRunner:
             return
         if len(jobs) > self.max_jobs:
             raise ValueError("%d jobs exceed the batch capacity %d" % (len(jobs), self.max_jobs))
-        prec = self._prec([w.ctx for w, _ in jobs])
+        prec = self._prec([w.ctx for w, _ in jobs], "eloc_precision")
         arr = (N.QmcEvalJob * len(jobs))()
         for k, (w, e) in enumerate(jobs):
             if e.dtype != torch.float64 or e.numel() != w.nw or not e.is_contiguous():

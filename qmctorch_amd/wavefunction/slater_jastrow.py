@@ -46,6 +46,7 @@ class SlaterJastrow(WaveFunction):
         mix_mo: bool = False,
         orthogonalize_mo: bool = False,
         precision: str = "fp64",
+        eloc_precision: Optional[str] = None,
     ) -> None:
         super().__init__(mol.nelec, 3, kinetic, cuda)
         if backflow is not None:
@@ -55,9 +56,13 @@ class SlaterJastrow(WaveFunction):
                                       % kinetic)
         if not include_all_mo and isinstance(configs, str) and configs.startswith("cas("):
             raise ValueError("CAS calculation only possible with include_all_mo=True")
-        if precision not in _PREC:
+        if precision not in _PREC or (eloc_precision is not None and eloc_precision not in _PREC):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        # precision: psi, pdf and the Metropolis sweep; eloc_precision: kinetic and
+        # local energy (FP64 unless asked otherwise: FP32 AO values leave
+        # ~1e-5 Ha/electron errors on rare walkers, FP64 costs little there)
         self.precision = precision
+        self.eloc_precision = eloc_precision or "fp64"
         self.mol = mol
         self.atoms = mol.atoms
         self.natom = mol.natom
@@ -172,7 +177,7 @@ class SlaterJastrow(WaveFunction):
         return pos.detach().to(torch.float64).contiguous()
 
     def _eval(self, pos: torch.Tensor, mode: int, precision: Optional[str] = None):
-        precision = precision or self.precision
+        precision = precision or (self.precision if mode == N.QMC_EVAL_PSI else self.eloc_precision)
         x = self._check_pos(pos)
         nw = x.shape[0]
         plan = self.device_plan()
