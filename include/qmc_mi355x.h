@@ -94,6 +94,11 @@ typedef struct QmcPlan QmcPlan;
  *               float64 otherwise (FP64 runs and every local-energy evaluation)
  *   ci          fc.weight             [nci]         (slater_jastrow.py:179-191)
  *   w_ee, w_en  Pade weights          [1] or NULL   (pade_jastrow_kernel.py:28)
+ * and the same values gathered into the order the kernels read them, so that
+ * no table lookup sits between a kernel and a parameter value:
+ *   grp_exp     bas_exp[grp_prim_src[j]]  [sum of grp_nprim]  float64
+ *   grp_expf    the same in float32
+ *   entry_w     mo_w row of AO entry i (mo_w[gao_mu[i], :])  [nentry*nmo], type of mo_w
  */
 typedef struct QmcParams {
     const double* bas_exp;
@@ -102,6 +107,9 @@ typedef struct QmcParams {
     const double* ci;
     const double* w_ee;
     const double* w_en;
+    const double* grp_exp;
+    const float* grp_expf;
+    const void* entry_w;
 } QmcParams;
 
 /* Outputs of qmc_wf_eval; any pointer may be NULL.  All float64, [nw]. */

@@ -50,6 +50,7 @@ class QmcParams(ctypes.Structure):
     _fields_ = [
         ("bas_exp", ctypes.c_void_p), ("atom_coords", ctypes.c_void_p), ("mo_w", ctypes.c_void_p),
         ("ci", ctypes.c_void_p), ("w_ee", ctypes.c_void_p), ("w_en", ctypes.c_void_p),
+        ("grp_exp", ctypes.c_void_p), ("grp_expf", ctypes.c_void_p), ("entry_w", ctypes.c_void_p),
     ]
 
 

@@ -151,7 +151,7 @@ namespace {
 int check_common(const QmcPlan* plan, const QmcParams* prm, int64_t nw) {
     if (!plan || !prm) return fail(QMC_ERR_ARG, "null plan or params");
     if (nw < 0) return fail(QMC_ERR_ARG, "negative walker count");
-    if (!prm->bas_exp || !prm->atom_coords || !prm->mo_w || !prm->ci)
+    if (!prm->bas_exp || !prm->atom_coords || !prm->mo_w || !prm->ci || !prm->grp_exp || !prm->grp_expf)
         return fail(QMC_ERR_ARG, "missing parameter pointer");
     if (plan->t.use_ee && !prm->w_ee) return fail(QMC_ERR_ARG, "e-e Jastrow weight missing");
     if (plan->t.use_en && !prm->w_en) return fail(QMC_ERR_ARG, "e-n Jastrow weight missing");
@@ -167,6 +167,7 @@ int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision,
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
     if (!pos || !out) return fail(QMC_ERR_ARG, "null pos/out");
+    if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
     if (nw == 0) return QMC_OK;
     KArgs a;
     a.t = plan->t;
@@ -190,6 +191,7 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
     if (rc) return rc;
     if (!wk || !sw || !wk->pos || !wk->logabs || !wk->sign) return fail(QMC_ERR_ARG, "null walker state");
     if (sw->keep && sw->keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
+    if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
     if (nw == 0 || sw->nstep <= 0) return QMC_OK;
     SArgs a;
     a.t = plan->t;
@@ -209,7 +211,8 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
 
 int qmc_ao_eval(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw, double* ao,
                 double* dao, double* d2ao, qmc_stream_t stream) {
-    if (!plan || !params || !params->bas_exp || !params->atom_coords) return fail(QMC_ERR_ARG, "null argument");
+    if (!plan || !params || !params->bas_exp || !params->atom_coords || !params->grp_exp || !params->grp_expf)
+        return fail(QMC_ERR_ARG, "null argument");
     if (!pos || !ao) return fail(QMC_ERR_ARG, "null pos/ao");
     if (nw == 0) return QMC_OK;
     AArgs a;
@@ -406,6 +409,7 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
         if (!x.walkers.pos || !x.walkers.logabs || !x.walkers.sign) return fail(QMC_ERR_ARG, "null walker state");
+        if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.sweep.keep && x.sweep.keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
         if (x.nw > 0 && x.sweep.nstep > 0) live.push_back(x);
     }
@@ -465,6 +469,7 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
         if (!x.pos) return fail(QMC_ERR_ARG, "null pos");
+        if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.nw > 0) live.push_back(x);
     }
     if (live.empty()) return QMC_OK;

@@ -98,7 +98,7 @@ __device__ __forceinline__ void cart_poly(T x, T y, T z, int kx, int ky, int kz,
 //     AO = Y sum K R,  grad AO = (sum K R) grad Y + Y (sum K R q) r,
 //     lap AO = Y sum K R (n(n+1)/r^2 - 2a(n+1)/r + a^2 + 2Lq) + (sum K R) lap Y
 //
-// sink(mu, ao, gx, gy, gz, lap) is called once per AO entry; entries of one
+// sink(i, mu, ao, gx, gy, gz, lap) is called once per AO entry i (AO mu); entries of one
 // AO that were split across groups are delivered separately (all consumers
 // are linear in the AO).  The primitive count of a group selects a fully
 // unrolled instance (no predicated slots).
@@ -113,14 +113,27 @@ __device__ __forceinline__ const QMC_CONST double* coeff_table<double>(const Tab
     return t.gao_K;
 }
 
+// exponents gathered in group-primitive order, in the evaluation precision
+template <typename T>
+__device__ __forceinline__ const T* group_exponents(const QmcParams& prm);
+template <>
+__device__ __forceinline__ const float* group_exponents<float>(const QmcParams& prm) {
+    return prm.grp_expf;
+}
+template <>
+__device__ __forceinline__ const double* group_exponents<double>(const QmcParams& prm) {
+    return prm.grp_exp;
+}
+
 template <typename T, int RAD, bool DERIV, int NP, typename Sink>
-__device__ __forceinline__ void ao_group(const Tables& t, int g, const double* __restrict__ bas_exp, T dx, T dy,
-                                         T dz, T r2, T r, T ir, Sink& sink) {
+__device__ __forceinline__ void ao_group(const Tables& t, int g, const QmcParams& prm, T dx, T dy, T dz, T r2, T r,
+                                         T ir, Sink& sink) {
     const int p0 = t.grp_prim0[g];
+    const T* gexp = group_exponents<T>(prm);
     T E[NP], A[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const T a = (T)ldc(bas_exp, t.grp_prim_src[p0 + p]);
+        const T a = ldc(gexp, p0 + p);
         A[p] = a;
         E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
     }
@@ -147,10 +160,10 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
             if (DERIV) {
                 const T L = T(kx + ky + kz);
                 const T c = T(-2) * s1 * Y;
-                sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
+                sink(i, t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
                      Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
             } else {
-                sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+                sink(i, t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
             }
         } else {
             const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
@@ -171,9 +184,9 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
             const T ao = Y * s0;
             if (DERIV) {
                 const T c = Y * s1;
-                sink(t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
+                sink(i, t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
             } else {
-                sink(t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
+                sink(i, t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
             }
         }
     }
@@ -183,9 +196,9 @@ __device__ __forceinline__ void ao_group(const Tables& t, int g, const double* _
 // mixed-precision local energy: tight core primitives cancel in lap AO near the
 // nucleus, which FP32 cannot resolve).
 template <typename T, int RAD, bool DERIV, bool TIGHT = false, typename Sink>
-__device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict__ bas_exp,
-                                       const double* __restrict__ atom_xyz, double ex, double ey, double ez,
+__device__ __forceinline__ void ao_row(const Tables& t, const QmcParams& prm, double ex, double ey, double ez,
                                        Sink&& sink) {
+    const double* atom_xyz = prm.atom_coords;
     int cur_atom = -1;
     double dxd = 0, dyd = 0, dzd = 0, r2d = 0, rd = 0, ird = 0;
     T dx = 0, dy = 0, dz = 0, r2 = 0, r = 0, ir = 0;
@@ -217,9 +230,9 @@ __device__ __forceinline__ void ao_row(const Tables& t, const double* __restrict
 #define QMC_AO_CASE(K)                                                                               \
     case K:                                                                                          \
         if (tight)                                                                                   \
-            ao_group<double, RAD, DERIV, K>(t, g, bas_exp, dxd, dyd, dzd, r2d, rd, ird, sink);       \
+            ao_group<double, RAD, DERIV, K>(t, g, prm, dxd, dyd, dzd, r2d, rd, ird, sink);           \
         else                                                                                         \
-            ao_group<T, RAD, DERIV, K>(t, g, bas_exp, dx, dy, dz, r2, r, ir, sink);                  \
+            ao_group<T, RAD, DERIV, K>(t, g, prm, dx, dy, dz, r2, r, ir, sink);                      \
         break;
             QMC_AO_CASE(1) QMC_AO_CASE(2) QMC_AO_CASE(3) QMC_AO_CASE(4) QMC_AO_CASE(5) QMC_AO_CASE(6)
             QMC_AO_CASE(7) QMC_AO_CASE(8) QMC_AO_CASE(9) QMC_AO_CASE(10) QMC_AO_CASE(11) QMC_AO_CASE(12)

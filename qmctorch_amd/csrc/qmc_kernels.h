@@ -72,7 +72,8 @@ __device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs, i
     return __builtin_amdgcn_readfirstlane(lo);
 }
 
-// MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95).
+// MO row accumulation sink: mo[m] += ao * W[mu, m]  (molecular_orbitals.py:79-95), with W
+// read from the entry-ordered copy (row i = W row of the entry's AO).
 // WT is the storage type of W, AT the accumulator type; AO values may arrive in
 // either precision (FP64 for tight exponent groups in the mixed local energy).
 template <typename WT, int NM, bool DERIV, typename AT = WT>
@@ -85,8 +86,8 @@ struct MoSink {
     AT* gz;
     AT* lp;
     template <typename V>
-    __device__ __forceinline__ void operator()(int mu, V ao, V ax, V ay, V az, V al) const {
-        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)mu * nmo;
+    __device__ __forceinline__ void operator()(int i, int, V ao, V ax, V ay, V az, V al) const {
+        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)i * nmo;  // W row of AO entry i
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             if (m < nmo) {
@@ -369,8 +370,8 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
     AT mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = AT(0);
-    MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.mo_w, t.nmo, mo, gx, gy, gz, lp};
-    ao_row<T, RAD, ELOC, ELOC>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+    MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.entry_w, t.nmo, mo, gx, gy, gz, lp};
+    ao_row<T, RAD, ELOC, ELOC>(t, a.p, x, y, z, sink);
 
     double kee, ken, jgx, jgy, jgz, jlap, vpot;
     jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vpot);
@@ -439,8 +440,8 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     T mo[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
-    MoSink<T, NM, false> sink{(const T*)prm.mo_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
-    ao_row<T, RAD, false>(t, prm.bas_exp, prm.atom_coords, x, y, z, sink);
+    MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
+    ao_row<T, RAD, false>(t, prm, x, y, z, sink);
     const double kj = (double)jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
     if (t.identity_conf) {
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void ao_kernel(AArgs a) {
     double* ao = a.ao;
     double* dao = a.dao;
     double* d2ao = a.d2ao;
-    auto sink = [&](int mu, double val, double ax, double ay, double az, double al) {
+    auto sink = [&](int, int mu, double val, double ax, double ay, double az, double al) {
         if (!v) return;
         ao[row + mu] += val;
         if (DERIV) {
@@ -599,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void ao_kernel(AArgs a) {
             if (d2ao) d2ao[row + mu] += al;
         }
     };
-    ao_row<double, RAD, DERIV>(t, a.p.bas_exp, a.p.atom_coords, p[0], p[1], p[2], sink);
+    ao_row<double, RAD, DERIV>(t, a.p, p[0], p[1], p[2], sink);
 }
 
 // Adjoint rows of log|psi| (see qmc_logpsi_grad_rows in the header).
@@ -727,14 +728,14 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(GArgs a) {
     {
         double* aor = a.ao_rows + row * t.nao;
         const double* W = (const double*)a.p.mo_w;
-        auto sink = [&](int mu, double val, double, double, double, double) {
+        auto sink = [&](int, int mu, double val, double, double, double, double) {
             if (v) aor[mu] += val;
             const QMC_CONST double* wr = (const QMC_CONST double*)W + (size_t)mu * t.nmo;
 #pragma unroll
             for (int m = 0; m < NM; ++m)
                 if (m < t.nmo) mo[m] += val * wr[m];
         };
-        ao_row<double, RAD, false>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+        ao_row<double, RAD, false>(t, a.p, x, y, z, sink);
     }
     double ld[QMC_MAXCONF], sd[QMC_MAXCONF], om[QMC_MAXCONF], gam[NM];
 #pragma unroll
@@ -976,7 +977,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
         const double* W = (const double*)a.p.mo_w;
         double* aor = a.ao_rows ? a.ao_rows + row * t.nao : nullptr;
         double* bor = a.b_rows ? a.b_rows + row * t.nao : nullptr;
-        auto sink = [&](int mu, double ao, double ax, double ay, double az, double al) {
+        auto sink = [&](int, int mu, double ao, double ax, double ay, double az, double al) {
             if (v && aor) aor[mu] += ao;
             if (v && bor) bor[mu] += -0.5 * (al + 2.0 * (g[0] * ax + g[1] * ay + g[2] * az) + l2 * ao);
             const QMC_CONST double* wr = (const QMC_CONST double*)W + (size_t)mu * t.nmo;
@@ -992,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
                 }
             }
         };
-        ao_row<double, RAD, true>(t, a.p.bas_exp, a.p.atom_coords, x, y, z, sink);
+        ao_row<double, RAD, true>(t, a.p, x, y, z, sink);
     }
     double bk[NM];
 #pragma unroll

@@ -80,7 +80,11 @@ class AtomicOrbitals(nn.Module):
         prm = N.QmcParams()
         be = self.bas_exp.detach().to(torch.float64).contiguous()
         ac = self.atom_coords.detach().to(torch.float64).contiguous()
+        src = torch.as_tensor(dp.tables.grp_prim_src.astype(np.int64), device=be.device)
+        gexp = be.index_select(0, src)
+        gexpf = gexp.to(torch.float32)
         prm.bas_exp, prm.atom_coords = be.data_ptr(), ac.data_ptr()
+        prm.grp_exp, prm.grp_expf = gexp.data_ptr(), gexpf.data_ptr()
         N.check(N.load().qmc_ao_eval(dp.handle, prm, x.data_ptr(), nw, ao.data_ptr(), N.ptr(dao), N.ptr(d2ao),
                                      N.current_stream()))
         dt = torch.get_default_dtype()

@@ -151,19 +151,37 @@ class SlaterJastrow(WaveFunction):
         self._plan_exp_version = ver
         return self._plan
 
+    def _gather_index(self):
+        """Device index tensors of the current plan: exponent of every group
+        primitive (into bas_exp) and AO of every entry (into the rows of W)."""
+        plan = self.device_plan()
+        if getattr(self, "_gather_for", None) is not plan:
+            tab = self._plan_tables
+            dev = self.ao.bas_exp.device
+            self._gather = (torch.as_tensor(tab.grp_prim_src.astype(np.int64), device=dev),
+                            torch.as_tensor(tab.gao_mu.astype(np.int64), device=dev))
+            self._gather_for = plan
+        return self._gather
+
     def _params(self, precision: str):
         """Parameter views handed to the kernels (all tiny device tensors)."""
         ee, en = self._jastrow_terms()
         f64 = torch.float64
         W = self.mo.weight()[:, : self.highest_occ_mo]
         W = W.detach().to(torch.float32 if precision == "mixed" else f64).contiguous()
+        bas_exp = self.ao.bas_exp.detach().to(f64).contiguous()
+        src, mu = self._gather_index()
+        gexp = bas_exp.index_select(0, src)
         keep = {
-            "bas_exp": self.ao.bas_exp.detach().to(f64).contiguous(),
+            "bas_exp": bas_exp,
             "atom_coords": self.ao.atom_coords.detach().to(f64).contiguous(),
             "mo_w": W,
             "ci": self.fc.weight.detach().reshape(-1).to(f64).contiguous(),
             "w_ee": ee.jastrow_kernel.weight.detach().to(f64).contiguous() if ee is not None else None,
             "w_en": en.jastrow_kernel.weight.detach().to(f64).contiguous() if en is not None else None,
+            "grp_exp": gexp,
+            "grp_expf": gexp.to(torch.float32),
+            "entry_w": W.index_select(0, mu).contiguous(),
         }
         p = N.QmcParams()
         for k, v in keep.items():
