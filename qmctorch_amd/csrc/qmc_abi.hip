@@ -350,7 +350,7 @@ template <typename JobT>
 int group_jobs(const JobT* jobs, int njobs, std::vector<Group>& groups) {
     for (int j = 0; j < njobs; ++j) {
         const Tables& t = jobs[j].plan->t;
-        const int bk = nm_bucket(t);
+        const int bk = nm_bucket_fine(t);
         if (bk < 0) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
         Group* g = nullptr;
         for (auto& x : groups)

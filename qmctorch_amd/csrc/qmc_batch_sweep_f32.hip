@@ -16,11 +16,12 @@ static void go(const SJob* jobs, int njobs, int nwaves, hipStream_t s) {
 template <>
 int launch_sweep_batch_f32<QMC_RAD>(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s) {
     switch (bucket) {
-        case 2: go<2, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
-        case 4: go<4, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
-        case 8: go<8, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
-        case 16: go<16, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
-        case 24: go<24, QMC_RAD>(jobs, njobs, nwaves, s); return 0;
+#define QMC_CASE(K)                  \
+    case K:                          \
+        go<K, QMC_RAD>(jobs, njobs, nwaves, s); \
+        return 0;
+        QMC_FINE_BUCKETS(QMC_CASE)
+#undef QMC_CASE
         default: return -1;
     }
 }

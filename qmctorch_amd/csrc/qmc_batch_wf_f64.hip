@@ -20,11 +20,12 @@ static void go(const WJob* jobs, int njobs, int nwaves, int mode, hipStream_t s)
 template <>
 int launch_wf_batch_f64<QMC_RAD>(const WJob* jobs, int njobs, int nwaves, int bucket, int mode, hipStream_t s) {
     switch (bucket) {
-        case 2: go<2, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
-        case 4: go<4, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
-        case 8: go<8, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
-        case 16: go<16, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
-        case 24: go<24, QMC_RAD>(jobs, njobs, nwaves, mode, s); return 0;
+#define QMC_CASE(K)                  \
+    case K:                          \
+        go<K, QMC_RAD>(jobs, njobs, nwaves, mode, s); \
+        return 0;
+        QMC_FINE_BUCKETS(QMC_CASE)
+#undef QMC_CASE
         default: return -1;
     }
 }

@@ -17,6 +17,21 @@ inline int nm_bucket(const Tables& t) {
     return -1;
 }
 
+// register-array bound of the psi / local-energy / sweep kernels: exact up to
+// 8 occupied orbitals per spin, so the lane elimination and the MO rows carry
+// no idle columns, then 12, 16, 24
+inline int nm_bucket_fine(const Tables& t) {
+    int need = t.nmo > t.nup ? t.nmo : t.nup;
+    need = need > t.ndown ? need : t.ndown;
+    if (need < 1) need = 1;
+    if (need <= 8) return need;
+    if (need <= 12) return 12;
+    if (need <= 16) return 16;
+    if (need <= 24) return 24;
+    return -1;
+}
+#define QMC_FINE_BUCKETS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(12) X(16) X(24)
+
 inline dim3 grid_for(const Tables& t, int64_t nw) {
     const int G = 64 / t.nelec;
     const int64_t waves = (nw + G - 1) / G;

@@ -18,12 +18,13 @@ static void go(const KArgs& a, int mode, hipStream_t s) {
 
 template <int RAD>
 static int bucket(const KArgs& a, int mode, hipStream_t s) {
-    switch (nm_bucket(a.t)) {
-        case 2: go<2, RAD>(a, mode, s); return 0;
-        case 4: go<4, RAD>(a, mode, s); return 0;
-        case 8: go<8, RAD>(a, mode, s); return 0;
-        case 16: go<16, RAD>(a, mode, s); return 0;
-        case 24: go<24, RAD>(a, mode, s); return 0;
+    switch (nm_bucket_fine(a.t)) {
+#define QMC_CASE(K)           \
+    case K:                   \
+        go<K, RAD>(a, mode, s); \
+        return 0;
+        QMC_FINE_BUCKETS(QMC_CASE)
+#undef QMC_CASE
         default: return -1;
     }
 }
