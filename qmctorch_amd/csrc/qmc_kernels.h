@@ -117,27 +117,38 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
     const int Ne = t.nelec;
     const double wee = (t.use_ee && prm.w_ee) ? ldc(prm.w_ee, 0) : 0.0;
     const double wen = (t.use_en && prm.w_en) ? ldc(prm.w_en, 0) : 0.0;
-    for (int j = 0; j < Ne; ++j) {
-        const double xj = __shfl(x, L.wbase + j);
-        const double yj = __shfl(y, L.wbase + j);
-        const double zj = __shfl(z, L.wbase + j);
-        if (j != L.e) {
-            const double dx = x - xj, dy = y - yj, dz = z - zj;
-            const double r = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ir = 1.0 / r;
-            if (DERIV) vpot += 0.5 * ir;
-            if (t.use_ee) {
-                const bool same = (j < t.nup) == (L.e < t.nup);
-                const double w0 = same ? 0.25 : 0.5;
-                const double den = 1.0 / (1.0 + wee * r);
-                kee += w0 * r * den;
-                if (DERIV) {
-                    const double kp = w0 * den * den;
-                    const double kpp = -2.0 * w0 * wee * den * den * den;
-                    gx += kp * dx * ir;
-                    gy += kp * dy * ir;
-                    gz += kp * dz * ir;
-                    lap += kpp + 2.0 * kp * ir;
+    // partner coordinates fetched four electrons at a time (independent shuffles)
+    for (int j0 = 0; j0 < Ne; j0 += 4) {
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int jl = L.wbase + (j0 + u < Ne ? j0 + u : 0);
+            px[u] = __shfl(x, jl);
+            py[u] = __shfl(y, jl);
+            pz[u] = __shfl(z, jl);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u;
+            const double xj = px[u], yj = py[u], zj = pz[u];
+            if (j < Ne && j != L.e) {
+                const double dx = x - xj, dy = y - yj, dz = z - zj;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double ir = 1.0 / r;
+                if (DERIV) vpot += 0.5 * ir;
+                if (t.use_ee) {
+                    const bool same = (j < t.nup) == (L.e < t.nup);
+                    const double w0 = same ? 0.25 : 0.5;
+                    const double den = 1.0 / (1.0 + wee * r);
+                    kee += w0 * r * den;
+                    if (DERIV) {
+                        const double kp = w0 * den * den;
+                        const double kpp = -2.0 * w0 * wee * den * den * den;
+                        gx += kp * dx * ir;
+                        gy += kp * dy * ir;
+                        gz += kp * dz * ir;
+                        lap += kpp + 2.0 * kp * ir;
+                    }
                 }
             }
         }

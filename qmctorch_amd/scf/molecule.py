@@ -6,9 +6,11 @@ of the reference keep working.  Differences, all forced by the offline image:
 
 * no SCF program (pyscf/ADF are absent): the basis comes from the published
   basis sets in ``basis_data`` laid out as pyscf.py:104-252 lays them out, and
-  the MO coefficients come from an s-only RHF (``rhf.py``) when the basis has
-  only s functions, otherwise from a seeded random orthonormal matrix, or from
-  the caller (``mo_coeffs=``);
+  the MO coefficients come from this package's RHF (``rhf.py``, integrals in
+  ``integrals.py``) for closed-shell molecules up to RHF_MAX_NAO AOs, in the
+  pure-d space as pyscf's spherical SCF, expressed on the AO layer's Cartesian
+  functions; otherwise from a seeded random orthonormal matrix
+  (``mo_coeffs='random'``), or from the caller (``mo_coeffs=<array>``);
 * ``load=`` reads ``.npz`` files converted from the reference's HDF5 molecule
   caches (``tests/golden``), since h5py is not installed for this interpreter.
 """
@@ -21,13 +23,33 @@ import numpy as np
 from .. import log
 from ..utils.constants import ANGS2BOHR
 from .basis import build_basis, normalize_columns, random_orthonormal_mos
-from .rhf import rhf_s
 
 ATOMIC_NUMBER = {
     "H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9,
     "Ne": 10, "Na": 11, "Mg": 12, "Al": 13, "Si": 14, "P": 15, "S": 16, "Cl": 17,
     "Ar": 18,
 }
+
+
+# largest AO count for which Molecule runs its own RHF by default (the integral
+# code is numpy; larger systems get seeded random orthonormal orbitals)
+RHF_MAX_NAO = 64
+_RHF_CACHE = {}
+
+
+def _cached_rhf(mol, basis):
+    """RHF of the molecule in the AO layer's basis (pure d space), cached per
+    process by geometry, basis and electron count."""
+    from .rhf import pure_d_transform, rhf
+
+    key = (tuple(mol.atoms), np.asarray(mol.atom_coords, dtype=np.float64).round(12).tobytes(),
+           mol.basis_name, mol.nelec)
+    if key not in _RHF_CACHE:
+        log.info("  Running RHF (own integrals, {0} AOs)", basis.nao)
+        _RHF_CACHE[key] = rhf(basis, mol.atom_coords, mol.atomic_number, mol.nelec,
+                              ao_transform=pure_d_transform(basis))
+    e, C = _RHF_CACHE[key]
+    return e, C.copy()
 
 
 class Molecule:
@@ -88,15 +110,13 @@ class Molecule:
     # ------------------------------------------------------------------ setup
     def _set_mos(self, mo_coeffs, seed: int) -> None:
         b = self.basis
-        only_s = not np.any(b.bas_kx + b.bas_ky + b.bas_kz)
         if isinstance(mo_coeffs, str):
             mode = mo_coeffs
             if mode == "auto":
-                mode = "rhf" if only_s else "random"
+                closed = self.nelec % 2 == 0 and self.nup == self.ndown
+                mode = "rhf" if closed and b.nao <= RHF_MAX_NAO else "random"
             if mode == "rhf":
-                if not only_s:
-                    raise ValueError("the built-in RHF handles s-only bases")
-                e, C = rhf_s(b, self.atom_coords, self.atomic_number, self.nelec)
+                e, C = _cached_rhf(self, b)
                 b.TotalEnergy = e
                 b.mos = normalize_columns(C)
             elif mode == "random":

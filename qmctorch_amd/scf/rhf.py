@@ -1,4 +1,8 @@
-"""Restricted Hartree-Fock for bases made only of s-type Gaussians.
+"""Restricted Hartree-Fock: ``rhf`` for s, p, d Gaussian bases (integrals from
+integrals.py) and ``rhf_s``, the closed-form s-only version kept as an
+independent check.
+
+Original notes of the s-only solver:
 
 pyscf (used by the reference's calculator, qmctorch/scf/calculator/pyscf.py:78-101)
 is unavailable offline.  For s-only bases (e.g. H2/STO-3G) the one- and
@@ -111,3 +115,98 @@ def rhf_s(basis, atom_coords, charges, nelec: int, maxiter: int = 100, tol: floa
         e_old = e_elec
     C = C * np.where(C[0:1, :] < 0, -1.0, 1.0)
     return e_elec + vnn, C
+
+
+def nuclear_repulsion(atom_coords, charges) -> float:
+    coords = np.asarray(atom_coords, dtype=np.float64)
+    vnn = 0.0
+    for a in range(len(charges)):
+        for b in range(a + 1, len(charges)):
+            vnn += charges[a] * charges[b] / np.sqrt(((coords[a] - coords[b]) ** 2).sum())
+    return float(vnn)
+
+
+def rhf(basis, atom_coords, charges, nelec: int, maxiter: int = 200, tol: float = 1e-11, ao_transform=None):
+    """Closed-shell Roothaan-Hall SCF with DIIS (Pulay, Chem. Phys. Lett. 73, 393
+    (1980)) in the AO basis of the AO layer (s, p, d Cartesian Gaussians; the
+    integrals of integrals.py).  ``ao_transform`` (nao x m) restricts the
+    variational space to its columns (e.g. the five pure d combinations).
+    Returns (E_total, C (nao x nmo)); columns are the canonical orbitals in
+    increasing orbital energy."""
+    from .integrals import ao_integrals
+
+    if nelec % 2:
+        raise ValueError("closed-shell RHF needs an even electron count")
+    S, T, V, eri = ao_integrals(basis, atom_coords, charges)
+    H = T + V
+    X0 = np.eye(S.shape[0]) if ao_transform is None else np.asarray(ao_transform, dtype=np.float64)
+    Sx = X0.T @ S @ X0
+    w, U = np.linalg.eigh(Sx)
+    keep = w > 1e-10 * w.max()
+    X = X0 @ (U[:, keep] / np.sqrt(w[keep])[None, :])
+    nocc = nelec // 2
+    e, Cp = np.linalg.eigh(X.T @ H @ X)
+    C = X @ Cp
+    P = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    e_old = None
+    focks, errs = [], []
+    for _ in range(maxiter):
+        J = np.einsum("ijkl,kl->ij", eri, P, optimize=True)
+        K = np.einsum("ikjl,kl->ij", eri, P, optimize=True)
+        F = H + J - 0.5 * K
+        e_elec = 0.5 * float((P * (H + F)).sum())
+        err = X.T @ (F @ P @ S - S @ P @ F) @ X
+        focks.append(F)
+        errs.append(err)
+        if len(focks) > 8:
+            focks.pop(0)
+            errs.pop(0)
+        if e_old is not None and abs(e_elec - e_old) < tol and np.abs(err).max() < 1e-8:
+            break
+        e_old = e_elec
+        if len(focks) >= 2:
+            m = len(focks)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(m):
+                    B[i, j] = float((errs[i] * errs[j]).sum())
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            try:
+                c = np.linalg.solve(B, rhs)[:m]
+                F = sum(ci * Fi for ci, Fi in zip(c, focks))
+            except np.linalg.LinAlgError:
+                pass
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        P = 2.0 * C[:, :nocc] @ C[:, :nocc].T
+    else:
+        raise RuntimeError("RHF did not converge in %d iterations" % maxiter)
+    C = C * np.where(C[np.argmax(np.abs(C), axis=0), np.arange(C.shape[1])] < 0, -1.0, 1.0)[None, :]
+    return e_elec + nuclear_repulsion(atom_coords, charges), C
+
+
+def pure_d_transform(basis) -> np.ndarray:
+    """Columns spanning the AO space with each Cartesian d shell reduced to its
+    five pure (l = 2) combinations xy, yz, xz, xx - yy, 2zz - xx - yy (the
+    variational space of an SCF run with spherical d functions, pyscf's default
+    for the bases used here); other AOs pass through.  nao x nmo."""
+    nao = int(basis.nao)
+    cols = []
+    d_aos = set()
+    for (_, l, _, a0) in getattr(basis, "shell_info", []):
+        if l == 2:
+            xx, xy, xz, yy, yz, zz = range(a0, a0 + 6)
+            d_aos.update(range(a0, a0 + 6))
+            for v in ({xy: 1.0}, {yz: 1.0}, {xz: 1.0}, {xx: 1.0, yy: -1.0}, {zz: 2.0, xx: -1.0, yy: -1.0}):
+                c = np.zeros(nao)
+                for k, w in v.items():
+                    c[k] = w
+                cols.append(c)
+    for mu in range(nao):
+        if mu not in d_aos:
+            c = np.zeros(nao)
+            c[mu] = 1.0
+            cols.append(c)
+    return np.stack(cols, 1)
