@@ -63,6 +63,18 @@ def _restatement(mol, wf_dev):
                                     jastrow_en=1.0 if en is not None else None)
 
 
+def measured_traffic(args, nw, steps_per_launch):
+    """HBM bytes per sweep launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by a plain copy of known size), when they were
+    taken on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "sweep_kernel_%s_%d_traffic.json" % (args.system, nw))
+    if not os.path.isfile(path) or args.precision != "mixed" or steps_per_launch != 50:
+        return None, None
+    with open(path) as f:
+        d = json.load(f)
+    return d["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
+
+
 def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=300, warm=2):
     """Reference baselines: the oracle restatement of the reference Metropolis
     step (host proposal RNG, per-step host<->device copies, FP64 ATen ops) timed
@@ -409,6 +421,7 @@ def main():
     err = {"max": float(du.max()), "mean": float(du.mean()), "eloc_precision": wf.eloc_precision}
     sweep_vs64 = sweep_precision_effect(mol, wf, nw) if args.precision == "mixed" else None
 
+    traffic, traffic_src = measured_traffic(args, nw, steps_per_launch)
     res = {
         "metric": METRIC,
         "value": value,
@@ -440,7 +453,8 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "sweep_kernel",
             "launch_ms": launch_s * 1e3,
             "flops_per_walker_step": F + 4 * T,
