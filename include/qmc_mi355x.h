@@ -161,6 +161,17 @@ typedef struct QmcSweepDesc {
 int qmc_plan_create(const QmcSystemDesc* desc, QmcPlan** plan);
 int qmc_plan_destroy(QmcPlan* plan);
 
+/* Kernels generated for one basis layout (a row with every group, entry and
+ * Cartesian power a compile-time constant) are built into the library for the
+ * benchmark molecules; a plan whose layout and register bucket match one uses
+ * them for the mixed-precision sweep and the FP64 local energy.  No
+ * counterpart in the reference (the layout there is a set of index tensors).
+ * qmc_plan_set_specialized(plan, 0) makes the plan use the table-driven
+ * kernels only (for comparisons); qmc_plan_specialization returns the
+ * layout's name, or NULL when the plan uses the table-driven kernels. */
+int qmc_plan_set_specialized(QmcPlan* plan, int32_t enable);
+const char* qmc_plan_specialization(const QmcPlan* plan);
+
 /* psi / log|psi| / kinetic / local energy of nw walkers.
  * Replaces SlaterJastrow.forward (slater_jastrow.py:243-286), WaveFunction.pdf
  * (wf_base.py:241-247), SlaterJastrow.kinetic_energy_jacobi

@@ -94,6 +94,8 @@ class QmcEvalJob(ctypes.Structure):
 EXPORTS = {
     "qmc_plan_create": ([ctypes.POINTER(QmcSystemDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "qmc_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "qmc_plan_set_specialized": ([ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
+    "qmc_plan_specialization": ([ctypes.c_void_p], ctypes.c_char_p),
     "qmc_wf_eval": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                      ctypes.c_int64, ctypes.POINTER(QmcWfOut), ctypes.c_void_p], ctypes.c_int),
     "qmc_ao_eval": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,

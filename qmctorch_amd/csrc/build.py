@@ -46,22 +46,36 @@ def _obj_dir() -> str:
 def compile_one(unit, extra=()) -> subprocess.CompletedProcess:
     src, tag, defs = unit
     obj = os.path.join(_obj_dir(), tag + ".o")
-    cmd = [HIPCC, *FLAGS, *defs, *extra, "-c", os.path.join(HERE, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *defs, *extra, "-c", src if os.path.isabs(src) else os.path.join(HERE, src), "-o", obj]
     return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def spec_units():
+    """Units of the kernels generated for the benchmark molecules' basis layouts
+    (written into _build/spec by specialize.py) plus their registry."""
+    if ROOT not in sys.path:
+        sys.path.insert(0, ROOT)
+    from qmctorch_amd.csrc import specialize
+
+    files = specialize.write_sources(os.path.join(_obj_dir(), "spec"))
+    return [(f, "spec_" + os.path.basename(f)[:-4], []) for f in files]
 
 
 def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) -> str:
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    jobs = jobs or min(len(UNITS), os.cpu_count() or 4, 16)
+    units = UNITS + spec_units()
+    # the largest units first, so the parallel build ends sooner
+    units = sorted(units, key=lambda u: 0 if u[0].startswith(os.path.join(_obj_dir(), "spec")) else 1)
+    jobs = jobs or min(len(units), os.cpu_count() or 4, 16)
     with ThreadPoolExecutor(jobs) as ex:
-        results = list(ex.map(lambda u: compile_one(u, extra), UNITS))
-    for unit, r in zip(UNITS, results):
+        results = list(ex.map(lambda u: compile_one(u, extra), units))
+    for unit, r in zip(units, results):
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("hipcc failed on %s" % unit[1])
         if verbose or resource_usage:
             sys.stderr.write(r.stderr)
-    objs = [os.path.join(_obj_dir(), u[1] + ".o") for u in UNITS]
+    objs = [os.path.join(_obj_dir(), u[1] + ".o") for u in units]
     tmp = LIB + ".tmp%d" % os.getpid()
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

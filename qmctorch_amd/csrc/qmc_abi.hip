@@ -44,6 +44,8 @@ __global__ void philox_kernel(const uint32_t* in, uint32_t* out, int64_t n) {
 // =========================================================================== plan
 struct QmcPlan {
     Tables t;
+    const SpecEntry* spec = nullptr;  // kernels generated for this basis layout, if any
+    bool use_spec = true;
     std::vector<void*> allocs;
     ~QmcPlan() {
         for (void* p : allocs) (void)hipFree(p);
@@ -59,6 +61,43 @@ struct QmcPlan {
         return (const X*)d;
     }
 };
+
+namespace {
+
+// FNV-1a over the layout fields a generated AO row bakes in (specialize.py
+// computes the same value over the same fields, 32-bit little-endian words).
+struct Fnv {
+    uint64_t h = 0xCBF29CE484222325ull;
+    void word(int32_t v) {
+        const uint32_t u = (uint32_t)v;
+        for (int b = 0; b < 4; ++b) {
+            h ^= (u >> (8 * b)) & 0xFFu;
+            h *= 0x100000001B3ull;
+        }
+    }
+    void words(const int32_t* v, int n) {
+        for (int i = 0; i < n; ++i) word(v ? v[i] : 0);
+    }
+};
+
+uint64_t layout_signature(const QmcSystemDesc* d) {
+    Fnv f;
+    f.word(d->radial_type);
+    f.word(d->natom);
+    f.word(d->ngroup);
+    f.words(d->grp_atom, d->ngroup);
+    f.words(d->grp_nprim, d->ngroup);
+    f.words(d->grp_prim0, d->ngroup);
+    f.words(d->grp_ao0, d->ngroup);
+    f.words(d->grp_nao, d->ngroup);
+    f.words(d->grp_tight, d->ngroup);
+    f.word(d->nentry);
+    f.words(d->gao_mu, d->nentry);
+    f.words(d->gao_k, d->nentry);
+    return f.h;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -131,6 +170,7 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.ci_up = (decltype(t.ci_up))p->upload(d->ci_up, d->nci);
         t.ci_dn = (decltype(t.ci_dn))p->upload(d->ci_dn, d->nci);
         if (!t.conf_dn) t.conf_dn = t.conf_up;
+        p->spec = find_spec(layout_signature(d), nm_bucket_fine(t));
     } catch (const std::exception& ex) {
         delete p;
         return fail(QMC_ERR_HIP, ex.what());
@@ -142,6 +182,16 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
 int qmc_plan_destroy(QmcPlan* plan) {
     delete plan;
     return QMC_OK;
+}
+
+int qmc_plan_set_specialized(QmcPlan* plan, int32_t enable) {
+    if (!plan) return fail(QMC_ERR_ARG, "null plan");
+    plan->use_spec = enable != 0;
+    return QMC_OK;
+}
+
+const char* qmc_plan_specialization(const QmcPlan* plan) {
+    return (plan && plan->spec && plan->use_spec) ? plan->spec->name : nullptr;
 }
 
 }  // extern "C"
@@ -178,6 +228,10 @@ int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision,
     hipStream_t s = (hipStream_t)stream;
     const bool sto = plan->t.radial == QMC_RADIAL_STO;
     (void)sto;
+    if (plan->spec && plan->use_spec && plan->spec->wf(a, precision, mode, s) == 0) {
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     rc = precision == QMC_PREC_FP64 ? QMC_BY_RAD(launch_wf_f64, a.t.radial, a, mode, s)
                                     : QMC_BY_RAD(launch_wf_f32, a.t.radial, a, mode, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
@@ -202,6 +256,10 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
     hipStream_t s = (hipStream_t)stream;
     const bool sto = plan->t.radial == QMC_RADIAL_STO;
     (void)sto;
+    if (plan->spec && plan->use_spec && plan->spec->sweep(a, precision, s) == 0) {
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     rc = precision == QMC_PREC_FP64 ? QMC_BY_RAD(launch_sweep_f64, a.t.radial, a, s)
                                     : QMC_BY_RAD(launch_sweep_f32, a.t.radial, a, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
@@ -341,6 +399,7 @@ int take_slot(QmcBatch* b, int& slot) {
 
 struct Group {
     int rad, bucket;
+    const SpecEntry* spec;
     std::vector<int> idx;
 };
 
@@ -352,11 +411,12 @@ int group_jobs(const JobT* jobs, int njobs, std::vector<Group>& groups) {
         const Tables& t = jobs[j].plan->t;
         const int bk = nm_bucket_fine(t);
         if (bk < 0) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+        const SpecEntry* sp = jobs[j].plan->use_spec ? jobs[j].plan->spec : nullptr;
         Group* g = nullptr;
         for (auto& x : groups)
-            if (x.rad == t.radial && x.bucket == bk) g = &x;
+            if (x.rad == t.radial && x.bucket == bk && x.spec == sp) g = &x;
         if (!g) {
-            groups.push_back(Group{t.radial, bk, {}});
+            groups.push_back(Group{t.radial, bk, sp, {}});
             g = &groups.back();
         }
         g->idx.push_back(j);
@@ -424,6 +484,7 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
     SJob* dev = (SJob*)(b->dev + slot * b->slot_bytes);
     struct Launch {
         int rad, bucket, first, n, nwaves;
+        const SpecEntry* spec;
     };
     std::vector<Launch> launches;
     int k = 0;
@@ -443,11 +504,12 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
             waves += (x.nw + G - 1) / G;
         }
         if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
-        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves});
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.spec});
     }
     hipStream_t s = (hipStream_t)stream;
     QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(SJob), hipMemcpyHostToDevice, s));
     for (const auto& L : launches) {
+        if (L.spec && L.spec->sweep_batch(dev + L.first, L.n, L.nwaves, precision, s) == 0) continue;
         const int r = precision == QMC_PREC_FP64
                           ? QMC_BY_RAD(launch_sweep_batch_f64, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, s)
                           : QMC_BY_RAD(launch_sweep_batch_f32, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, s);
@@ -483,6 +545,7 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
     WJob* dev = (WJob*)(b->dev + slot * b->slot_bytes);
     struct Launch {
         int rad, bucket, first, n, nwaves;
+        const SpecEntry* spec;
     };
     std::vector<Launch> launches;
     int k = 0;
@@ -502,11 +565,12 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
             waves += (x.nw + G - 1) / G;
         }
         if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
-        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves});
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.spec});
     }
     hipStream_t s = (hipStream_t)stream;
     QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(WJob), hipMemcpyHostToDevice, s));
     for (const auto& L : launches) {
+        if (L.spec && L.spec->wf_batch(dev + L.first, L.n, L.nwaves, precision, mode, s) == 0) continue;
         const int r =
             precision == QMC_PREC_FP64
                 ? QMC_BY_RAD(launch_wf_batch_f64, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, mode, s)

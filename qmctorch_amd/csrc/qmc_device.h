@@ -125,70 +125,83 @@ __device__ __forceinline__ const double* group_exponents<double>(const QmcParams
     return prm.grp_exp;
 }
 
-template <typename T, int RAD, bool DERIV, int NP, typename Sink>
-__device__ __forceinline__ void ao_group(const Tables& t, int g, const QmcParams& prm, T dx, T dy, T dz, T r2, T r,
-                                         T ir, Sink& sink) {
-    const int p0 = t.grp_prim0[g];
+// exp(-a r^2) (or exp(-a r)) of the NP primitives of a group whose exponents
+// start at p0 of the gathered exponent list
+template <typename T, int RAD, int NP>
+__device__ __forceinline__ void group_prims(const QmcParams& prm, int p0, T r2, T r, T (&E)[NP], T (&A)[NP]) {
     const T* gexp = group_exponents<T>(prm);
-    T E[NP], A[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const T a = ldc(gexp, p0 + p);
         A[p] = a;
         E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
     }
-    const QMC_CONST T* Ktab = coeff_table<T>(t);
+}
+
+// AO entry i (AO mu, Cartesian powers kx, ky, kz) of a group from its
+// primitive exponentials; hands value, gradient and Laplacian to the sink.
+template <typename T, int RAD, bool DERIV, int NP, typename Sink>
+__device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx, int ky, int kz, const T (&E)[NP],
+                                         const T (&A)[NP], T dx, T dy, T dz, T r2, T r, T ir, Sink& sink) {
+    T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
+    cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
+    const QMC_CONST T* Kp = coeff_table<T>(t) + (size_t)i * QMC_MAXP;
+    T s0 = 0, s1 = 0, s2 = 0;
+    if (RAD == QMC_RADIAL_GTO_PURE) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const T ke = Kp[p] * E[p];
+            s0 += ke;
+            if (DERIV) {
+                s1 += ke * A[p];
+                s2 += ke * A[p] * A[p];
+            }
+        }
+        const T ao = Y * s0;
+        if (DERIV) {
+            const T L = T(kx + ky + kz);
+            const T c = T(-2) * s1 * Y;
+            sink(i, mu, ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
+                 Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
+        } else {
+            sink(i, mu, ao, T(0), T(0), T(0), T(0));
+        }
+    } else {
+        const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
+        const T L = T(kx + ky + kz);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const T n = (T)Np[p];
+            const int ni = (int)Np[p];
+            const T kr = Kp[p] * ipow(r, ni) * E[p];
+            s0 += kr;
+            if (DERIV) {
+                const T a = A[p];
+                const T q = n * ir * ir - a * ir;
+                s1 += kr * q;
+                s2 += kr * (n * (n + T(1)) * ir * ir - T(2) * a * (n + T(1)) * ir + a * a + T(2) * L * q);
+            }
+        }
+        const T ao = Y * s0;
+        if (DERIV) {
+            const T c = Y * s1;
+            sink(i, mu, ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
+        } else {
+            sink(i, mu, ao, T(0), T(0), T(0), T(0));
+        }
+    }
+}
+
+template <typename T, int RAD, bool DERIV, int NP, typename Sink>
+__device__ __forceinline__ void ao_group(const Tables& t, int g, const QmcParams& prm, T dx, T dy, T dz, T r2, T r,
+                                         T ir, Sink& sink) {
+    T E[NP], A[NP];
+    group_prims<T, RAD, NP>(prm, t.grp_prim0[g], r2, r, E, A);
     const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
     for (int i = i0; i < i1; ++i) {
         const int kk = t.gao_k[i];
-        const int kx = kk & 15, ky = (kk >> 4) & 15, kz = (kk >> 8) & 15;
-        T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
-        cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
-        const QMC_CONST T* Kp = Ktab + (size_t)i * QMC_MAXP;
-        T s0 = 0, s1 = 0, s2 = 0;
-        if (RAD == QMC_RADIAL_GTO_PURE) {
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                const T ke = Kp[p] * E[p];
-                s0 += ke;
-                if (DERIV) {
-                    s1 += ke * A[p];
-                    s2 += ke * A[p] * A[p];
-                }
-            }
-            const T ao = Y * s0;
-            if (DERIV) {
-                const T L = T(kx + ky + kz);
-                const T c = T(-2) * s1 * Y;
-                sink(i, t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz,
-                     Y * (T(4) * r2 * s2 - (T(6) + T(4) * L) * s1) + s0 * Yl);
-            } else {
-                sink(i, t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
-            }
-        } else {
-            const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
-            const T L = T(kx + ky + kz);
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                const T n = (T)Np[p];
-                const int ni = (int)Np[p];
-                const T kr = Kp[p] * ipow(r, ni) * E[p];
-                s0 += kr;
-                if (DERIV) {
-                    const T a = A[p];
-                    const T q = n * ir * ir - a * ir;
-                    s1 += kr * q;
-                    s2 += kr * (n * (n + T(1)) * ir * ir - T(2) * a * (n + T(1)) * ir + a * a + T(2) * L * q);
-                }
-            }
-            const T ao = Y * s0;
-            if (DERIV) {
-                const T c = Y * s1;
-                sink(i, t.gao_mu[i], ao, s0 * Yx + c * dx, s0 * Yy + c * dy, s0 * Yz + c * dz, Y * s2 + s0 * Yl);
-            } else {
-                sink(i, t.gao_mu[i], ao, T(0), T(0), T(0), T(0));
-            }
-        }
+        ao_entry<T, RAD, DERIV, NP>(t, i, t.gao_mu[i], kk & 15, (kk >> 4) & 15, (kk >> 8) & 15, E, A, dx, dy, dz, r2,
+                                    r, ir, sink);
     }
 }
 
@@ -240,6 +253,39 @@ __device__ __forceinline__ void ao_row(const Tables& t, const QmcParams& prm, do
             default: break;
         }
     }
+}
+
+// AO evaluation policies of the kernels: AoGeneric walks the group tables
+// above; a generated policy (qmc_spec_*.hip) walks one basis layout with every
+// group, entry, primitive offset and Cartesian power a compile-time constant,
+// built from the same atom_geo / group_prims / ao_entry pieces in the same
+// order, so both give the same values.
+struct AoGeneric {
+    template <typename T, int RAD, bool DERIV, bool TIGHT, typename Sink>
+    static __device__ __forceinline__ void row(const Tables& t, const QmcParams& prm, double ex, double ey, double ez,
+                                               Sink&& sink) {
+        ao_row<T, RAD, DERIV, TIGHT>(t, prm, ex, ey, ez, sink);
+    }
+};
+
+template <typename T>
+struct AtomGeo {
+    T dx, dy, dz, r2, r, ir;
+};
+
+template <typename T, int RAD>
+__device__ __forceinline__ AtomGeo<T> atom_geo(double dxd, double dyd, double dzd) {
+    AtomGeo<T> g;
+    g.dx = (T)dxd;
+    g.dy = (T)dyd;
+    g.dz = (T)dzd;
+    g.r2 = g.dx * g.dx + g.dy * g.dy + g.dz * g.dz;
+    g.r = g.ir = T(0);
+    if (RAD == QMC_RADIAL_STO) {
+        g.r = fsqrt(g.r2);
+        g.ir = T(1) / g.r;
+    }
+    return g;
 }
 
 // ------------------------------------------------------- lane-row elimination

@@ -369,7 +369,7 @@ struct KArgs {
 };
 
 // psi / kinetic / local energy of the walkers of one wavefront.
-template <typename T, int NM, int RAD, bool ELOC>
+template <typename T, int NM, int RAD, bool ELOC, typename AO = AoGeneric>
 __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
@@ -382,7 +382,7 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = AT(0);
     MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.entry_w, t.nmo, mo, gx, gy, gz, lp};
-    ao_row<T, RAD, ELOC, ELOC>(t, a.p, x, y, z, sink);
+    AO::template row<T, RAD, ELOC, ELOC>(t, a.p, x, y, z, sink);
 
     double kee, ken, jgx, jgy, jgz, jlap, vpot;
     jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vpot);
@@ -413,11 +413,11 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
     }
 }
 
-template <typename T, int NM, int RAD, bool ELOC>
+template <typename T, int NM, int RAD, bool ELOC, typename AO = AoGeneric>
 __global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;  // whole wavefront beyond the batch
-    wf_walkers<T, NM, RAD, ELOC>(a, L);
+    wf_walkers<T, NM, RAD, ELOC, AO>(a, L);
 }
 
 // Many systems in one launch: job j owns global waves [wave0[j], wave0[j+1]).
@@ -426,7 +426,7 @@ struct WJob {
     int64_t wave0;
 };
 
-template <typename T, int NM, int RAD, bool ELOC>
+template <typename T, int NM, int RAD, bool ELOC, typename AO = AoGeneric>
 __global__ __launch_bounds__(kBlock) void wf_batch_kernel(const WJob* __restrict__ jobs, int njobs, int nwaves) {
     const int wave = global_wave();
     if (wave >= nwaves) return;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void wf_batch_kernel(const WJob* __restrict
     const KArgs a = jobs[j].a;
     const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    wf_walkers<T, NM, RAD, ELOC>(a, L);
+    wf_walkers<T, NM, RAD, ELOC, AO>(a, L);
 }
 
 struct SArgs {
@@ -445,14 +445,14 @@ struct SArgs {
     int64_t nw;
 };
 
-template <typename T, int NM, int RAD>
+template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
                                               double y, double z, double& sgn) {
     T mo[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
     MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
-    ao_row<T, RAD, false>(t, prm, x, y, z, sink);
+    AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
     const double kj = (double)jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
     if (t.identity_conf) {
@@ -466,7 +466,7 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
 // state in registers (sampler/metropolis.py:134-165, 256-298).
-template <typename T, int NM, int RAD>
+template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
     const int Ne = t.nelec;
@@ -475,7 +475,7 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     double sgn;
     double lp;
     if (a.sw.init_logpsi) {
-        lp = eval_logpsi<T, NM, RAD>(t, a.p, L, x, y, z, sgn);
+        lp = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, x, y, z, sgn);
     } else {
         lp = a.wk.logabs[L.wclamp];
         sgn = a.wk.sign[L.wclamp];
@@ -515,7 +515,7 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        double lpn = eval_logpsi<T, NM, RAD>(t, a.p, L, xn, yn, zn, sgn_n);
+        double lpn = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, xn, yn, zn, sgn_n);
         if (!(lpn > lp_floor)) lpn = lp_floor;
         // acceptance uniform: counter slot 0x80000000 of the same walker and step;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
@@ -551,11 +551,11 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     }
 }
 
-template <typename T, int NM, int RAD>
+template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    sweep_walkers<T, NM, RAD>(a, L);
+    sweep_walkers<T, NM, RAD, AO>(a, L);
 }
 
 // Sweeps of many systems (each with its own walkers, step count and seed) in
@@ -566,7 +566,7 @@ struct SJob {
     int64_t wave0;
 };
 
-template <typename T, int NM, int RAD>
+template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restrict__ jobs, int njobs, int nwaves) {
     const int wave = global_wave();
     if (wave >= nwaves) return;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restr
     const SArgs a = jobs[j].a;
     const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    sweep_walkers<T, NM, RAD>(a, L);
+    sweep_walkers<T, NM, RAD, AO>(a, L);
 }
 
 // AO values and derivatives in the reference layout (atomic_orbitals.py:578-609).

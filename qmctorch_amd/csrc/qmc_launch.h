@@ -57,6 +57,22 @@ int launch_wf_batch_f64(const WJob* jobs, int njobs, int nwaves, int bucket, int
 
 inline dim3 grid_for_waves(int nwaves) { return dim3((unsigned)((nwaves + (kBlock / 64) - 1) / (kBlock / 64))); }
 
+// Kernels built for one basis layout (generated units, specialize.py).  Each
+// launcher returns 0, or -1 when the layout has no instance for that
+// precision / mode (the caller then uses the generic kernels).
+struct SpecEntry {
+    uint64_t signature;  // layout_signature of the layout
+    int nm;              // register bucket the kernels were built for
+    const char* name;
+    int (*sweep)(const SArgs& a, int precision, hipStream_t s);
+    int (*sweep_batch)(const SJob* jobs, int njobs, int nwaves, int precision, hipStream_t s);
+    int (*wf)(const KArgs& a, int precision, int mode, hipStream_t s);  // FP64 local energy
+    int (*wf_batch)(const WJob* jobs, int njobs, int nwaves, int precision, int mode, hipStream_t s);
+};
+
+// the generated layout with this signature and bucket, or nullptr
+const SpecEntry* find_spec(uint64_t signature, int nm);
+
 #define QMC_BY_RAD(fn, rad, ...) ((rad) == QMC_RADIAL_STO ? fn<QMC_RADIAL_STO>(__VA_ARGS__) : fn<QMC_RADIAL_GTO_PURE>(__VA_ARGS__))
 
 }  // namespace qmc

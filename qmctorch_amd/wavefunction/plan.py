@@ -268,3 +268,14 @@ class DevicePlan:
                 self.handle = ctypes.c_void_p()
         except Exception:
             pass
+
+    @property
+    def specialization(self):
+        """Name of the basis layout whose generated kernels this plan uses, or None."""
+        name = self._lib.qmc_plan_specialization(self.handle)
+        return name.decode() if name else None
+
+    def set_specialized(self, enable: bool) -> None:
+        """Use the generated kernels of the plan's layout (default) or only the
+        table-driven ones."""
+        N.check(self._lib.qmc_plan_set_specialized(self.handle, 1 if enable else 0))
