@@ -98,7 +98,8 @@ typedef struct QmcPlan QmcPlan;
  * no table lookup sits between a kernel and a parameter value:
  *   grp_exp     bas_exp[grp_prim_src[j]]  [sum of grp_nprim]  float64
  *   grp_expf    the same in float32
- *   entry_w     mo_w row of AO entry i (mo_w[gao_mu[i], :])  [nentry*nmo], type of mo_w
+ *   entry_w     mo_w row of AO entry i (mo_w[gao_mu[i], :]) padded with zeros to
+ *               B = qmc_plan_bucket(plan) columns  [nentry*B], type of mo_w
  */
 typedef struct QmcParams {
     const double* bas_exp;
@@ -170,6 +171,10 @@ int qmc_plan_destroy(QmcPlan* plan);
  * kernels only (for comparisons); qmc_plan_specialization returns the
  * layout's name, or NULL when the plan uses the table-driven kernels. */
 int qmc_plan_set_specialized(QmcPlan* plan, int32_t enable);
+
+/* Register bucket of a plan: the column count of its kernels' MO rows
+ * (>= nmo, nup, ndown), i.e. the row stride of QmcParams.entry_w. */
+int qmc_plan_bucket(const QmcPlan* plan);
 const char* qmc_plan_specialization(const QmcPlan* plan);
 
 /* psi / log|psi| / kinetic / local energy of nw walkers.

@@ -96,6 +96,7 @@ EXPORTS = {
     "qmc_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "qmc_plan_set_specialized": ([ctypes.c_void_p, ctypes.c_int32], ctypes.c_int),
     "qmc_plan_specialization": ([ctypes.c_void_p], ctypes.c_char_p),
+    "qmc_plan_bucket": ([ctypes.c_void_p], ctypes.c_int),
     "qmc_wf_eval": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p,
                      ctypes.c_int64, ctypes.POINTER(QmcWfOut), ctypes.c_void_p], ctypes.c_int),
     "qmc_ao_eval": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,

@@ -190,6 +190,8 @@ int qmc_plan_set_specialized(QmcPlan* plan, int32_t enable) {
     return QMC_OK;
 }
 
+int qmc_plan_bucket(const QmcPlan* plan) { return plan ? nm_bucket_fine(plan->t) : -1; }
+
 const char* qmc_plan_specialization(const QmcPlan* plan) {
     return (plan && plan->spec && plan->use_spec) ? plan->spec->name : nullptr;
 }

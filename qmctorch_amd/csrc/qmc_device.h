@@ -54,8 +54,46 @@ struct Tables {
 };
 
 // ------------------------------------------------------------------ math
-__device__ __forceinline__ float fexp(float x) { return expf(x); }
+// exp(x) in FP32 as 2^(x log2 e) on the hardware exp2: log2 e = L_hi + L_lo,
+// y = x L_hi rounded, e = (x L_hi - y) + x L_lo (fma gives the rounding error
+// exactly), exp(x) = 2^y 2^e = 2^y (1 + e ln 2 + O(e^2)) with |e| < 2^-16 for
+// |x| < 128; error about 2 ulp.
+__device__ __forceinline__ float fexp(float x) {
+    const float L_hi = 1.44269502162933349609375f;  // log2(e) rounded to float
+    const float L_lo = 1.925963033500011e-8f;       // log2(e) - L_hi
+    const float y = x * L_hi;
+    const float e = fmaf(x, L_hi, -y) + x * L_lo;
+    const float p = __builtin_amdgcn_exp2f(y);
+    return fmaf(p, e * 0.693147182464599609375f, p);
+}
 __device__ __forceinline__ double fexp(double x) { return exp(x); }
+
+// 1/x: the hardware reciprocal (1 ulp) in FP32, a division in FP64
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
+
+// log(m) for m in [0.5, 1) (a frexp mantissa): with m' = m or 2m in
+// [1/sqrt 2, sqrt 2), z = (m' - 1)/(m' + 1), |z| < 0.172,
+// log m' = 2 atanh z = 2 (z + z^3/3 + ... + z^21/21) (truncation < 1e-19),
+// log m = log m' - [m' = 2m] log 2
+__device__ __forceinline__ double log_mantissa(double m) {
+    const bool up = m < 0.70710678118654752;
+    const double mm = up ? 2.0 * m : m;
+    const double z = (mm - 1.0) / (mm + 1.0);
+    const double s = z * z;
+    double p = 1.0 / 21.0;
+    p = fma(p, s, 1.0 / 19.0);
+    p = fma(p, s, 1.0 / 17.0);
+    p = fma(p, s, 1.0 / 15.0);
+    p = fma(p, s, 1.0 / 13.0);
+    p = fma(p, s, 1.0 / 11.0);
+    p = fma(p, s, 1.0 / 9.0);
+    p = fma(p, s, 1.0 / 7.0);
+    p = fma(p, s, 1.0 / 5.0);
+    p = fma(p, s, 1.0 / 3.0);
+    p = fma(p, s, 1.0);
+    return 2.0 * z * p - (up ? 0.6931471805599453 : 0.0);
+}
 __device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 
@@ -364,7 +402,7 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
         }
     }
     if (inversions & 1) sgn = -sgn;
-    logdet = singular ? -INFINITY : log(mant) + 0.6931471805599453 * (double)expo;
+    logdet = singular ? -INFINITY : log_mantissa(mant) + 0.6931471805599453 * (double)expo;
     trace_part = 0.0;
     if (WITH_B) {
 #pragma unroll

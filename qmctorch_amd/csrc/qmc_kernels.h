@@ -85,20 +85,20 @@ struct MoSink {
     AT* gy;
     AT* gz;
     AT* lp;
+    // row i of the entry-ordered W holds NM columns (zeros past nmo), so the
+    // accumulation needs no per-column test
     template <typename V>
     __device__ __forceinline__ void operator()(int i, int, V ao, V ax, V ay, V az, V al) const {
-        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)i * nmo;  // W row of AO entry i
+        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)i * NM;  // W row of AO entry i
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-            if (m < nmo) {
-                const AT w = (AT)wr[m];
-                mo[m] += (AT)ao * w;
-                if (DERIV) {
-                    gx[m] += (AT)ax * w;
-                    gy[m] += (AT)ay * w;
-                    gz[m] += (AT)az * w;
-                    lp[m] += (AT)al * w;
-                }
+            const AT w = (AT)wr[m];
+            mo[m] += (AT)ao * w;
+            if (DERIV) {
+                gx[m] += (AT)ax * w;
+                gy[m] += (AT)ay * w;
+                gz[m] += (AT)az * w;
+                lp[m] += (AT)al * w;
             }
         }
     }
@@ -205,7 +205,7 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
                     const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
                     const T r = fsqrt(dx * dx + dy * dy + dz * dz);
                     const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
-                    k += (j != L.e) ? w0 * r / (T(1) + wee * r) : T(0);
+                    k += (j != L.e) ? w0 * r * frcp(T(1) + wee * r) : T(0);
                 }
             }
         }
@@ -216,7 +216,7 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
             const T dx = (T)(x - t.jas_atoms[3 * A]), dy = (T)(y - t.jas_atoms[3 * A + 1]),
                     dz = (T)(z - t.jas_atoms[3 * A + 2]);
             const T r = fsqrt(dx * dx + dy * dy + dz * dz);
-            k += r / (T(1) + wen * r);
+            k += r * frcp(T(1) + wen * r);
         }
     }
     return k;

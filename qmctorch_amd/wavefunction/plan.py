@@ -270,6 +270,11 @@ class DevicePlan:
             pass
 
     @property
+    def bucket(self) -> int:
+        """Column count of the kernels' MO rows (row stride of the entry-ordered W)."""
+        return int(self._lib.qmc_plan_bucket(self.handle))
+
+    @property
     def specialization(self):
         """Name of the basis layout whose generated kernels this plan uses, or None."""
         name = self._lib.qmc_plan_specialization(self.handle)

@@ -172,6 +172,9 @@ class SlaterJastrow(WaveFunction):
         bas_exp = self.ao.bas_exp.detach().to(f64).contiguous()
         src, mu = self._gather_index()
         gexp = bas_exp.index_select(0, src)
+        nb = self.device_plan().bucket
+        went = torch.zeros(len(mu), nb, dtype=W.dtype, device=W.device)
+        went[:, : W.shape[1]] = W.index_select(0, mu)
         keep = {
             "bas_exp": bas_exp,
             "atom_coords": self.ao.atom_coords.detach().to(f64).contiguous(),
@@ -181,7 +184,7 @@ class SlaterJastrow(WaveFunction):
             "w_en": en.jastrow_kernel.weight.detach().to(f64).contiguous() if en is not None else None,
             "grp_exp": gexp,
             "grp_expf": gexp.to(torch.float32),
-            "entry_w": W.index_select(0, mu).contiguous(),
+            "entry_w": went,
         }
         p = N.QmcParams()
         for k, v in keep.items():
