@@ -238,8 +238,11 @@ def serve_per_request(requests: Sequence[VMCRequest], contexts: Dict[str, System
     out = []
     for req in sorted(requests, key=lambda r: (r.arrival, r.rid)):
         if realtime:
-            while clock() - t0 < req.arrival:
-                time.sleep(min(req.arrival - (clock() - t0), 1e-3))
+            while True:
+                dt = req.arrival - (clock() - t0)
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 1e-3))
         start = clock() - t0
         ctx = contexts[req.system]
         wf = ctx.wf
