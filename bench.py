@@ -75,7 +75,7 @@ def measured_traffic(args, nw, steps_per_launch):
     return d["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=300, warm=2):
+def cpu_baseline(mol, wf_dev, sample_walkers=1024, min_seconds=12.0, max_steps=3000, warm=2):
     """Reference baselines: the oracle restatement of the reference Metropolis
     step (host proposal RNG, per-step host<->device copies, FP64 ATen ops) timed
     on the host CPU cores and, on the same GPU, through torch-ROCm -- the
@@ -89,9 +89,15 @@ def cpu_baseline(mol, wf_dev, sample_walkers=1024, sample_steps=300, warm=2):
     torch.set_num_threads(cores)
     pos = helpers.positions(mol, sample_walkers, seed=3)
     O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0)
-    t0 = time.perf_counter()
-    O.metropolis(orc.pdf, pos.clone(), sample_steps, step_size=0.2, seed=0)
-    dt = time.perf_counter() - t0
+    # chunks of 50 steps until at least min_seconds of CPU work
+    sample_steps, dt = 0, 0.0
+    x = pos.clone()
+    while dt < min_seconds and sample_steps < max_steps:
+        t0 = time.perf_counter()
+        kept, _ = O.metropolis(orc.pdf, x, 50, step_size=0.2, seed=sample_steps)
+        dt += time.perf_counter() - t0
+        x = kept[-sample_walkers:].clone()
+        sample_steps += 50
     cpu = {"value": sample_walkers * sample_steps / dt, "unit": "walker-steps/s", "cores": cores,
            "kind": "port",
            "sample": "%d walkers x %d Metropolis steps of the same system, torch CPU float64 (%.1f s)"
@@ -316,7 +322,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the H2 / LiH / H2O-optimisation legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-system batched-launch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-loop leg")
-    ap.add_argument("--serving-rates", default="500", help="comma-separated Poisson rates (req/s)")
+    ap.add_argument("--serving-rates", default="50,500,2000", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) to scale HBM counters")
@@ -468,8 +474,10 @@ def main():
         rows = serving([float(r) for r in args.serving_rates.split(",")], args.serving_requests,
                        precision=args.precision)
         res["serving"] = rows
-        res["p50_ms"] = rows[0]["batched"]["p50_ms"]
-        res["p99_ms"] = rows[0]["batched"]["p99_ms"]
+        head = next((r for r in rows if r["rate_req_per_s"] == 500.0), rows[0])
+        res["p50_ms"] = head["batched"]["p50_ms"]
+        res["p99_ms"] = head["batched"]["p99_ms"]
+        res["p50_p99_at_rate_req_per_s"] = head["rate_req_per_s"]
     if args.copy_calibration:
         src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).uniform_()
         dst = torch.empty_like(src)
