@@ -94,7 +94,7 @@ __device__ __forceinline__ double log_mantissa(double m) {
     p = fma(p, s, 1.0);
     return 2.0 * z * p - (up ? 0.6931471805599453 : 0.0);
 }
-__device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // hardware, 1 ulp
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 
 template <typename T>
@@ -413,10 +413,11 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
 
 // Sum of v over the lanes [base, base+cnt) (result in every lane; cnt uniform bound).
 // Terms are fetched eight at a time (independent shuffles) and added in lane order.
-__device__ __forceinline__ double seg_sum(double v, int base, int cnt, int cnt_max) {
-    double s = 0.0;
+template <typename V>
+__device__ __forceinline__ V seg_sum(V v, int base, int cnt, int cnt_max) {
+    V s = V(0);
     for (int t0 = 0; t0 < cnt_max; t0 += 8) {
-        double x[8];
+        V x[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) x[u] = __shfl(v, base + (t0 + u < cnt ? t0 + u : 0));
 #pragma unroll

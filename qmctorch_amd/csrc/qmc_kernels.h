@@ -225,7 +225,7 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
 // Single configuration, single CI term: log|D_up D_dn| and its sign (ground state).
 template <int NM, typename T>
 __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& prm, const WalkerLane& L,
-                                              const T (&mo)[NM], double& logabs, double& sgn) {
+                                              const T (&mo)[NM], double logc0, double& logabs, double& sgn) {
     double a[NM], b[NM];
 #pragma unroll
     for (int j = 0; j < NM; ++j) a[j] = (double)mo[j];
@@ -236,7 +236,7 @@ __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& 
     const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
     const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
     const double c0 = ldc(prm.ci, 0);
-    logabs = lu + (t.ndown > 0 ? ldn : 0.0) + log(fabs(c0));
+    logabs = lu + (t.ndown > 0 ? ldn : 0.0) + logc0;
     sgn = su * (t.ndown > 0 ? sdn : 1.0) * (c0 < 0.0 ? -1.0 : 1.0);
 }
 
@@ -447,21 +447,22 @@ struct SArgs {
 
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
-                                              double y, double z, double& sgn) {
+                                              double y, double z, double logc0, double& sgn) {
     T mo[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
     MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
     AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
-    const double kj = (double)jastrow_value<T>(t, prm, L, x, y, z);
+    const T kj = jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
     if (t.identity_conf) {
-        slater_single<NM>(t, prm, L, mo, logabs, sgn);
+        slater_single<NM>(t, prm, L, mo, logc0, logabs, sgn);
     } else {
         double bk[NM], kin;
         slater_combine<NM, false>(t, prm, L, mo, bk, logabs, sgn, kin);
     }
-    return logabs + seg_sum(kj, L.wbase, t.nelec, t.nelec);
+    // ln J of the walker, summed in the bulk precision
+    return logabs + (double)seg_sum<T>(kj, L.wbase, t.nelec, t.nelec);
 }
 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
@@ -474,8 +475,9 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     double x = pp[0], y = pp[1], z = pp[2];
     double sgn;
     double lp;
+    const double logc0 = log(fabs(ldc(a.p.ci, 0)));  // single-configuration CI weight, once per launch
     if (a.sw.init_logpsi) {
-        lp = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, x, y, z, sgn);
+        lp = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, x, y, z, logc0, sgn);
     } else {
         lp = a.wk.logabs[L.wclamp];
         sgn = a.wk.sign[L.wclamp];
@@ -515,7 +517,7 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        double lpn = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, xn, yn, zn, sgn_n);
+        double lpn = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, xn, yn, zn, logc0, sgn_n);
         if (!(lpn > lp_floor)) lpn = lp_floor;
         // acceptance uniform: counter slot 0x80000000 of the same walker and step;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
