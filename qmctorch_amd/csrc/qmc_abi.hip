@@ -149,7 +149,6 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.grp_atom = (decltype(t.grp_atom))p->upload(d->grp_atom, d->ngroup);
         t.grp_nprim = (decltype(t.grp_nprim))p->upload(d->grp_nprim, d->ngroup);
         t.grp_prim0 = (decltype(t.grp_prim0))p->upload(d->grp_prim0, d->ngroup);
-        t.nprim_tot = nprim_tot;
         t.grp_prim_src = (decltype(t.grp_prim_src))p->upload(d->grp_prim_src, nprim_tot);
         t.grp_ao0 = (decltype(t.grp_ao0))p->upload(d->grp_ao0, d->ngroup);
         t.grp_nao = (decltype(t.grp_nao))p->upload(d->grp_nao, d->ngroup);

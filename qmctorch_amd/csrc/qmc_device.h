@@ -27,35 +27,9 @@ __device__ __forceinline__ X ldc(const X* p, size_t i) {
     return ((const __attribute__((address_space(4))) X*)p)[i];
 }
 
-// Bulk constants of the AO row (exponents, contraction coefficients, MO rows)
-// are read with buffer loads at wave-uniform offsets: they land in vector
-// registers, where a whole AO row's worth can be in flight without exhausting
-// the ~100 scalar registers (scalar loads of the same values were spilled to
-// vector lanes and reloaded one readlane at a time).  The descriptor bounds the
-// reads to the table (out-of-range offsets read as zero).
-struct Buf {
-    __amdgpu_buffer_rsrc_t r;
-};
-
-__device__ __forceinline__ Buf make_buf(const void* p, size_t nbytes) {
-    // word 3 of the gfx9-family raw buffer descriptor
-    return Buf{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)nbytes, 0x00020000)};
-}
-
-template <typename V>
-__device__ __forceinline__ V bld(const Buf& b, int index);
-template <>
-__device__ __forceinline__ float bld<float>(const Buf& b, int index) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, 0, index * 4, 0));
-}
-template <>
-__device__ __forceinline__ double bld<double>(const Buf& b, int index) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, 0, index * 8, 0));
-}
-
 struct Tables {
     int nelec, nup, ndown, natom, nao, nprim_ref, radial;
-    int ngroup, nentry, nmo, nci, nconf_up, nconf_dn, nconf_max, nsig_max, nprim_tot;
+    int ngroup, nentry, nmo, nci, nconf_up, nconf_dn, nconf_max, nsig_max;
     int use_ee, use_en, identity_conf;
     double vnn;
     const QMC_CONST int* grp_atom;
@@ -192,12 +166,11 @@ __device__ __forceinline__ const double* group_exponents<double>(const QmcParams
 // exp(-a r^2) (or exp(-a r)) of the NP primitives of a group whose exponents
 // start at p0 of the gathered exponent list
 template <typename T, int RAD, int NP>
-__device__ __forceinline__ void group_prims(const Tables& t, const QmcParams& prm, int p0, T r2, T r, T (&E)[NP],
-                                            T (&A)[NP]) {
-    const Buf bx = make_buf(group_exponents<T>(prm), (size_t)t.nprim_tot * sizeof(T));
+__device__ __forceinline__ void group_prims(const QmcParams& prm, int p0, T r2, T r, T (&E)[NP], T (&A)[NP]) {
+    const T* gexp = group_exponents<T>(prm);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        const T a = bld<T>(bx, p0 + p);
+        const T a = ldc(gexp, p0 + p);
         A[p] = a;
         E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
     }
@@ -210,13 +183,12 @@ __device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx,
                                          const T (&A)[NP], T dx, T dy, T dz, T r2, T r, T ir, Sink& sink) {
     T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
     cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
-    const Buf bk = make_buf((const void*)coeff_table<T>(t), (size_t)t.nentry * QMC_MAXP * sizeof(T));
-    const int k0 = i * QMC_MAXP;
+    const QMC_CONST T* Kp = coeff_table<T>(t) + (size_t)i * QMC_MAXP;
     T s0 = 0, s1 = 0, s2 = 0;
     if (RAD == QMC_RADIAL_GTO_PURE) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            const T ke = bld<T>(bk, k0 + p) * E[p];
+            const T ke = Kp[p] * E[p];
             s0 += ke;
             if (DERIV) {
                 s1 += ke * A[p];
@@ -233,14 +205,13 @@ __device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx,
             sink(i, mu, ao, T(0), T(0), T(0), T(0));
         }
     } else {
-        const Buf bn = make_buf((const void*)t.gao_n, (size_t)t.nentry * QMC_MAXP * sizeof(double));
+        const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
         const T L = T(kx + ky + kz);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            const double np_ = bld<double>(bn, k0 + p);
-            const T n = (T)np_;
-            const int ni = (int)np_;
-            const T kr = bld<T>(bk, k0 + p) * ipow(r, ni) * E[p];
+            const T n = (T)Np[p];
+            const int ni = (int)Np[p];
+            const T kr = Kp[p] * ipow(r, ni) * E[p];
             s0 += kr;
             if (DERIV) {
                 const T a = A[p];
@@ -263,7 +234,7 @@ template <typename T, int RAD, bool DERIV, int NP, typename Sink>
 __device__ __forceinline__ void ao_group(const Tables& t, int g, const QmcParams& prm, T dx, T dy, T dz, T r2, T r,
                                          T ir, Sink& sink) {
     T E[NP], A[NP];
-    group_prims<T, RAD, NP>(t, prm, t.grp_prim0[g], r2, r, E, A);
+    group_prims<T, RAD, NP>(prm, t.grp_prim0[g], r2, r, E, A);
     const int i0 = t.grp_ao0[g], i1 = i0 + t.grp_nao[g];
     for (int i = i0; i < i1; ++i) {
         const int kk = t.gao_k[i];

@@ -79,7 +79,7 @@ __device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs, i
 template <typename WT, int NM, bool DERIV, typename AT = WT>
 struct MoSink {
     const WT* __restrict__ W;
-    int nentry;
+    int nmo;
     AT* mo;
     AT* gx;
     AT* gy;
@@ -89,10 +89,10 @@ struct MoSink {
     // accumulation needs no per-column test
     template <typename V>
     __device__ __forceinline__ void operator()(int i, int, V ao, V ax, V ay, V az, V al) const {
-        const Buf bw = make_buf(W, (size_t)nentry * NM * sizeof(WT));  // W row of AO entry i at i * NM
+        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)i * NM;  // W row of AO entry i
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
-            const AT w = (AT)bld<WT>(bw, i * NM + m);
+            const AT w = (AT)wr[m];
             mo[m] += (AT)ao * w;
             if (DERIV) {
                 gx[m] += (AT)ax * w;
@@ -381,7 +381,7 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
     AT mo[NM], gx[NM], gy[NM], gz[NM], lp[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = lp[m] = AT(0);
-    MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.entry_w, t.nentry, mo, gx, gy, gz, lp};
+    MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.entry_w, t.nmo, mo, gx, gy, gz, lp};
     AO::template row<T, RAD, ELOC, ELOC>(t, a.p, x, y, z, sink);
 
     double kee, ken, jgx, jgy, jgz, jlap, vpot;
@@ -451,7 +451,7 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     T mo[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
-    MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nentry, mo, nullptr, nullptr, nullptr, nullptr};
+    MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
     AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
     const T kj = jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
@@ -465,29 +465,10 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     return logabs + (double)seg_sum<T>(kj, L.wbase, t.nelec, t.nelec);
 }
 
-// A pointer the compiler must treat as changed here (an empty asm claiming to
-// modify it): loads through it are issued again instead of their values being
-// kept alive in scalar registers from an earlier point.
-template <typename X>
-__device__ __forceinline__ X* opaque(X* ptr) {
-    asm volatile("" : "+s"(ptr));
-    return ptr;
-}
-
-__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
-    asm volatile("" : "+s"(v));
-    return v;
-}
-
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
 // state in registers (sampler/metropolis.py:134-165, 256-298).
-// The arguments (kernel arguments or a job-table entry, both read-only during
-// the launch) are read through the constant address space, re-read every step
-// (opaque), so that the ~60 pointers and counts they hold are not all kept in
-// scalar registers across the step loop, which would spill them to vector lanes.
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__device__ __forceinline__ void sweep_walkers(const QMC_CONST SArgs* ap, const WalkerLane& L) {
-    const SArgs& a = *ap;
+__device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
     const int Ne = t.nelec;
     double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
@@ -512,8 +493,7 @@ __device__ __forceinline__ void sweep_walkers(const QMC_CONST SArgs* ap, const W
         const uint64_t step = (uint64_t)(a.sw.step0 + s);
         // counter: (walker id lo, walker id hi, step, electron | stream<<16)
         const u4 ctr{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, (uint32_t)L.e};
-        // the key is re-derived each step: its 20 round keys are not held across the loop
-        const u4 rnd = philox4x32_10(ctr, opaque_u32(k0), opaque_u32(k1));
+        const u4 rnd = philox4x32_10(ctr, k0, k1);
         double nx, ny, nz;
         if (a.sw.uniform_move) {
             nx = a.sw.step_size * (2.0 * u01_closed0(rnd.x) - 1.0);
@@ -537,13 +517,11 @@ __device__ __forceinline__ void sweep_walkers(const QMC_CONST SArgs* ap, const W
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        const QMC_CONST SArgs* aq = opaque(ap);
-        double lpn = eval_logpsi<T, NM, RAD, AO>(aq->t, aq->p, L, xn, yn, zn, logc0, sgn_n);
+        double lpn = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, xn, yn, zn, logc0, sgn_n);
         if (!(lpn > lp_floor)) lpn = lp_floor;
         // acceptance uniform: counter slot 0x80000000 of the same walker and step;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
-        const u4 ua = philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u},
-                                    opaque_u32(k0), opaque_u32(k1));
+        const u4 ua = philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u}, k0, k1);
         const double U = u01_closed0(ua.x);
         const bool acc = 2.0 * (lpn - lp) >= (U > 0.0 ? (double)logf((float)U) : -INFINITY);
         if (acc) {
@@ -575,15 +553,11 @@ __device__ __forceinline__ void sweep_walkers(const QMC_CONST SArgs* ap, const W
     }
 }
 
-// Two waves per SIMD must fit (at most 256 VGPRs): at the usual walker counts
-// the launch has 1-2 waves per SIMD and the second one hides the first's latency.
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__global__ __launch_bounds__(kBlock, 2) void sweep_kernel(SArgs a) {
+__global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    // the argument is the only one, at offset 0 of the kernel-argument segment,
-    // which is read-only for the launch
-    sweep_walkers<T, NM, RAD, AO>((const QMC_CONST SArgs*)__builtin_amdgcn_kernarg_segment_ptr(), L);
+    sweep_walkers<T, NM, RAD, AO>(a, L);
 }
 
 // Sweeps of many systems (each with its own walkers, step count and seed) in
@@ -595,15 +569,14 @@ struct SJob {
 };
 
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__global__ __launch_bounds__(kBlock, 2) void sweep_batch_kernel(const SJob* __restrict__ jobs, int njobs,
-                                                                int nwaves) {
+__global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restrict__ jobs, int njobs, int nwaves) {
     const int wave = global_wave();
     if (wave >= nwaves) return;
     const int j = find_job(jobs, njobs, wave);
-    const QMC_CONST SArgs* ap = (const QMC_CONST SArgs*)&jobs[j].a;
-    const WalkerLane L = lane_layout_w(ap->t, ap->nw, wave - (int)jobs[j].wave0);
-    if ((int64_t)L.wave * L.G >= ap->nw) return;
-    sweep_walkers<T, NM, RAD, AO>(ap, L);
+    const SArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD, AO>(a, L);
 }
 
 // AO values and derivatives in the reference layout (atomic_orbitals.py:578-609).
