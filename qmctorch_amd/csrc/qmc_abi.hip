@@ -94,6 +94,16 @@ uint64_t layout_signature(const QmcSystemDesc* d) {
     f.word(d->nentry);
     f.words(d->gao_mu, d->nentry);
     f.words(d->gao_k, d->nentry);
+    // coefficients and r powers (literals of a generated row), as 32-bit words
+    for (const double* arr : {d->gao_K, d->gao_n}) {
+        const size_t n = (size_t)d->nentry * QMC_MAXP;
+        for (size_t k = 0; k < n; ++k) {
+            uint64_t u = 0;
+            if (arr) std::memcpy(&u, arr + k, sizeof(u));
+            f.word((int32_t)(uint32_t)(u & 0xFFFFFFFFu));
+            f.word((int32_t)(uint32_t)(u >> 32));
+        }
+    }
     return f.h;
 }
 

@@ -178,17 +178,20 @@ __device__ __forceinline__ void group_prims(const QmcParams& prm, int p0, T r2, 
 
 // AO entry i (AO mu, Cartesian powers kx, ky, kz) of a group from its
 // primitive exponentials; hands value, gradient and Laplacian to the sink.
-template <typename T, int RAD, bool DERIV, int NP, typename Sink>
-__device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx, int ky, int kz, const T (&E)[NP],
-                                         const T (&A)[NP], T dx, T dy, T dz, T r2, T r, T ir, Sink& sink) {
+// kcoef(p) is the contraction coefficient K_p of primitive p, rpow(p) its STO
+// r power (unused for Gaussians): read from the plan tables, or written into a
+// generated row as literals.
+template <typename T, int RAD, bool DERIV, int NP, typename KF, typename NF, typename Sink>
+__device__ __forceinline__ void ao_entry_with(KF kcoef, NF rpow, int i, int mu, int kx, int ky, int kz,
+                                              const T (&E)[NP], const T (&A)[NP], T dx, T dy, T dz, T r2, T r,
+                                              T ir, Sink& sink) {
     T Y, Yx = 0, Yy = 0, Yz = 0, Yl = 0;
     cart_poly<T, DERIV>(dx, dy, dz, kx, ky, kz, Y, Yx, Yy, Yz, Yl);
-    const QMC_CONST T* Kp = coeff_table<T>(t) + (size_t)i * QMC_MAXP;
     T s0 = 0, s1 = 0, s2 = 0;
     if (RAD == QMC_RADIAL_GTO_PURE) {
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            const T ke = Kp[p] * E[p];
+            const T ke = kcoef(p) * E[p];
             s0 += ke;
             if (DERIV) {
                 s1 += ke * A[p];
@@ -205,13 +208,13 @@ __device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx,
             sink(i, mu, ao, T(0), T(0), T(0), T(0));
         }
     } else {
-        const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
         const T L = T(kx + ky + kz);
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-            const T n = (T)Np[p];
-            const int ni = (int)Np[p];
-            const T kr = Kp[p] * ipow(r, ni) * E[p];
+            const double np_ = rpow(p);
+            const T n = (T)np_;
+            const int ni = (int)np_;
+            const T kr = kcoef(p) * ipow(r, ni) * E[p];
             s0 += kr;
             if (DERIV) {
                 const T a = A[p];
@@ -228,6 +231,15 @@ __device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx,
             sink(i, mu, ao, T(0), T(0), T(0), T(0));
         }
     }
+}
+
+template <typename T, int RAD, bool DERIV, int NP, typename Sink>
+__device__ __forceinline__ void ao_entry(const Tables& t, int i, int mu, int kx, int ky, int kz, const T (&E)[NP],
+                                         const T (&A)[NP], T dx, T dy, T dz, T r2, T r, T ir, Sink& sink) {
+    const QMC_CONST T* Kp = coeff_table<T>(t) + (size_t)i * QMC_MAXP;
+    const QMC_CONST double* Np = t.gao_n + (size_t)i * QMC_MAXP;
+    ao_entry_with<T, RAD, DERIV, NP>([&](int p) { return Kp[p]; }, [&](int p) { return Np[p]; }, i, mu, kx, ky, kz,
+                                     E, A, dx, dy, dz, r2, r, ir, sink);
 }
 
 template <typename T, int RAD, bool DERIV, int NP, typename Sink>

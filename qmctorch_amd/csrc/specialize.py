@@ -4,9 +4,9 @@ The table-driven AO row of qmc_device.h reads the group layout (atom,
 primitive count and offset, AO entries, Cartesian powers) from the plan on
 every group; each read is a scalar load whose result the next instruction
 waits for.  For the layouts of the benchmark molecules this module writes a
-row in which every one of those values is a compile-time constant (only
-exponents, coefficients, MO rows and positions are still read, at constant
-offsets the compiler can issue together), and instantiates the sweep, the
+row in which every one of those values, and the contraction coefficients,
+are compile-time constants (only exponents, MO rows and positions are still
+read, at constant offsets the compiler can issue together), and instantiates the sweep, the
 batched sweep and the (FP64) local-energy kernels with it.  At plan creation
 the library matches a plan to a generated layout by the signature computed
 here and in qmc_abi.hip over the same fields; other plans use the generic row.
@@ -42,6 +42,10 @@ def layout_signature(tab) -> int:
     h = _fnv(h, [len(tab.gao_mu)])
     h = _fnv(h, tab.gao_mu)
     h = _fnv(h, tab.gao_k)
+    # contraction coefficients and r powers are literals of the generated row:
+    # their float64 bit patterns, as pairs of 32-bit words
+    for arr in (tab.gao_K, tab.gao_n):
+        h = _fnv(h, np.ascontiguousarray(arr, dtype=np.float64).view(np.uint32).astype(np.int64))
     return h
 
 
@@ -78,10 +82,17 @@ def row_source(tab) -> str:
                        "group_prims<%s, RAD, %d>(prm, %d, %s.r2, %s.r, E, A);" % (T, npr, p0, geo, geo)]
                 for i in range(i0, i0 + nao):
                     k = int(tab.gao_k[i])
-                    out.append("ao_entry<%s, RAD, DERIV, %d>(t, %d, %d, %d, %d, %d, E, A, %s.dx, %s.dy, %s.dz, "
+                    kc = ", ".join(repr(float(v)) for v in tab.gao_K[i, :npr])
+                    nc = ", ".join(repr(float(v)) for v in tab.gao_n[i, :npr])
+                    out.append("{")
+                    out.append("    constexpr double K[%d] = {%s};" % (npr, kc))
+                    out.append("    constexpr double N[%d] = {%s};" % (npr, nc))
+                    out.append("    ao_entry_with<%s, RAD, DERIV, %d>([&](int p) { return (%s)K[p]; }, "
+                               "[&](int p) { return N[p]; }, %d, %d, %d, %d, %d, E, A, %s.dx, %s.dy, %s.dz, "
                                "%s.r2, %s.r, %s.ir, sink);"
-                               % (T, npr, i, int(tab.gao_mu[i]), k & 15, (k >> 4) & 15, (k >> 8) & 15,
+                               % (T, npr, T, i, int(tab.gao_mu[i]), k & 15, (k >> 4) & 15, (k >> 8) & 15,
                                   geo, geo, geo, geo, geo, geo))
+                    out.append("}")
                 return out
 
             lines.append("            // group %d: %d primitives from %d, entries %d..%d%s"
