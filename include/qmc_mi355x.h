@@ -196,6 +196,41 @@ int qmc_ao_eval(const QmcPlan* plan, const QmcParams* params, const double* pos,
 int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precision,
                          QmcWalkers* walkers, int64_t nw, const QmcSweepDesc* sweep, qmc_stream_t stream);
 
+/*
+ * Hamiltonian Monte Carlo (sampler/hamiltonian.py:80-201) on U = -ln psi^2:
+ * per step, momenta p ~ N(0, 1) per coordinate, E0 = U(q) + |p|^2/2, a
+ * leapfrog trajectory of traj_length position updates of step_size
+ * (half momentum steps at both ends), E1 = U(q') + |p'|^2/2, and the move is
+ * kept iff exp(E0 - E1) >= u, u ~ U[0,1) (a NaN energy rejects the move).
+ * grad U = -2 grad ln|psi| is evaluated on the device (no autograd).  Random
+ * numbers: Philox4x32-10 keyed by seed, counter (walker id, step, electron |
+ * 2^30) for the momenta (Box-Muller in FP64), (walker id, step, 2^31) for u.
+ * log|psi| of the walkers is evaluated at the start of every call (the stored
+ * walker logabs is not read); logabs/sign are written back.  Positions of the
+ * kept steps (keep_from / keep_every as in QmcSweepDesc) go to keep.
+ */
+typedef struct QmcHmcDesc {
+    int64_t step0;
+    int32_t nstep;
+    int32_t traj_length; /* L >= 1 */
+    double step_size;    /* epsilon */
+    uint64_t seed;
+    int64_t keep_from;
+    int32_t keep_every;
+    int32_t reserved;
+    double* keep; /* [nkeep * nw * 3*nelec] or NULL */
+} QmcHmcDesc;
+
+int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precision, QmcWalkers* walkers, int64_t nw,
+                  const QmcHmcDesc* hmc, qmc_stream_t stream);
+
+/* grad_i ln|psi| of every electron, drift [nw][nelec][3], with log|psi| and
+ * sign (either may be NULL), in the precision of the sampling kernels.
+ * Replaces SlaterJastrow.gradients_jacobi (slater_jastrow.py:346-447), which
+ * returns psi * drift (pdf=False) or 2 psi^2 * drift (pdf=True). */
+int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision, const double* pos, int64_t nw,
+                 double* drift, double* logabs, double* sign, qmc_stream_t stream);
+
 /* Adjoint of log|psi| with respect to the parameters, as per-walker rows
  * (AO values are recomputed here, not stored by the forward pass):
  *   ao_rows    [nw*nelec][nao]  AO values of every electron (reference AO layout)

@@ -743,6 +743,62 @@ def metropolis(pdf, pos, nstep, step_size=0.2, ntherm=-1, ndecor=1, proba="norma
     return torch.cat(kept), float(rate) / nstep
 
 
+# --------------------------------------------------------------------------- #
+# sampler/hamiltonian.py
+# --------------------------------------------------------------------------- #
+def grad_potential(pdf, q):
+    """grad of U = -log pdf by autograd (hamiltonian.py:44-77)."""
+    with torch.enable_grad():
+        x = q.detach().clone().requires_grad_(True)
+        val = -torch.log(pdf(x))
+        val.backward(torch.ones_like(val))
+    return x.grad.detach()
+
+
+def hmc_step(pdf, q_init, p, eps, L, u):
+    """One step of Hamiltonian._step (hamiltonian.py:143-201) with the momenta p
+    (Nw, 3Ne) and the uniforms u (Nw,) given: E = U + |p|^2/2 with U = -log
+    pdf; half momentum step, L-1 full (position, momentum) steps, a last
+    position step and a closing half momentum step; the move is rejected iff
+    exp(E_init - E_new) < u.  Returns (positions, accepted mask)."""
+    with torch.no_grad():
+        q = q_init.clone()
+        e_init = -torch.log(pdf(q)) + 0.5 * (p * p).sum(1)
+        p = p - 0.5 * eps * grad_potential(pdf, q)
+        for _ in range(L - 1):
+            q = q + eps * p
+            p = p - eps * grad_potential(pdf, q)
+        q = q + eps * p
+        p = p - 0.5 * eps * grad_potential(pdf, q)
+        p = -p
+        e_new = -torch.log(pdf(q)) + 0.5 * (p * p).sum(1)
+        rejected = torch.exp(e_init - e_new) < u
+        q[rejected] = q_init[rejected]
+    return q, ~rejected
+
+
+def hamiltonian(pdf, pos, nstep, step_size=0.2, L=10, ntherm=-1, ndecor=1, seed=None):
+    """Hamiltonian.__call__ (hamiltonian.py:80-141): momenta from torch.randn,
+    uniforms from torch.rand, in that order every step.  Returns (positions
+    kept, acceptance rate)."""
+    if seed is not None:
+        torch.manual_seed(seed)
+    if ntherm < 0:
+        ntherm = nstep + ntherm
+    q = pos.detach().clone()
+    kept, rate, idecor = [], 0.0, 0
+    for istep in range(nstep):
+        p = torch.randn(q.shape, dtype=q.dtype)
+        u = torch.rand(q.shape[0], dtype=q.dtype)
+        q, acc = hmc_step(pdf, q, p, step_size, L, u)
+        rate += float(acc.double().mean())
+        if istep >= ntherm:
+            if idecor % ndecor == 0:
+                kept.append(q.clone())
+            idecor += 1
+    return torch.cat(kept), rate / nstep
+
+
 def single_point(wf, pos):
     """Local energy batch and its statistics (solver_base.py:316-387)."""
     eloc = wf.local_energy(pos)
@@ -772,5 +828,8 @@ PARITY = {
     "JastrowEE/JastrowEN/combine_jastrow": "pinned: autograd gradient/Laplacian, cusp slope, permutation invariance",
     "kinetic_energy_jacobi/local_energy": "pinned: autograd Laplacian, H-atom and single-Gaussian closed forms, LiH golden fixture",
     "metropolis": "pinned statistically: exact eigenfunction (zero variance), detailed-balance moments",
+    "hmc_step/hamiltonian": "pinned: leapfrog of a Gaussian target against the closed-form linear map, exact "
+                            "energy change, Gaussian moments (tests/test_oracle_hmc.py)",
+    "grad_potential": "pinned: -2 grad ln|psi| against gradients_jacobi / psi and central differences",
     "manual_gradient": "pinned: finite differences of <E_L> weights",
 }

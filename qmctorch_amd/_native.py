@@ -77,6 +77,14 @@ class QmcSweepDesc(ctypes.Structure):
     ]
 
 
+class QmcHmcDesc(ctypes.Structure):
+    _fields_ = [
+        ("step0", ctypes.c_int64), ("nstep", ctypes.c_int32), ("traj_length", ctypes.c_int32),
+        ("step_size", ctypes.c_double), ("seed", ctypes.c_uint64), ("keep_from", ctypes.c_int64),
+        ("keep_every", ctypes.c_int32), ("reserved", ctypes.c_int32), ("keep", ctypes.c_void_p),
+    ]
+
+
 class QmcSweepJob(ctypes.Structure):
     _fields_ = [
         ("plan", ctypes.c_void_p), ("params", QmcParams), ("walkers", QmcWalkers), ("nw", ctypes.c_int64),
@@ -103,6 +111,10 @@ EXPORTS = {
                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "qmc_metropolis_sweep": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.POINTER(QmcWalkers),
                               ctypes.c_int64, ctypes.POINTER(QmcSweepDesc), ctypes.c_void_p], ctypes.c_int),
+    "qmc_hmc_sweep": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.POINTER(QmcWalkers),
+                       ctypes.c_int64, ctypes.POINTER(QmcHmcDesc), ctypes.c_void_p], ctypes.c_int),
+    "qmc_wf_drift": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64,
+                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "qmc_logpsi_grad_rows": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64,
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                               ctypes.c_void_p],

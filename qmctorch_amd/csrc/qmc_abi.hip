@@ -109,6 +109,21 @@ uint64_t layout_signature(const QmcSystemDesc* d) {
 
 }  // namespace
 
+namespace {
+int launch_hmc(const HArgs& a, int precision, hipStream_t s) {
+    const bool f64 = precision == QMC_PREC_FP64, sto = a.t.radial == QMC_RADIAL_STO;
+    if (a.t.nconf_max <= 1) {
+        if (f64) return sto ? launch_hmc_f64<QMC_RADIAL_STO, 1>(a, s) : launch_hmc_f64<QMC_RADIAL_GTO_PURE, 1>(a, s);
+        return sto ? launch_hmc_f32<QMC_RADIAL_STO, 1>(a, s) : launch_hmc_f32<QMC_RADIAL_GTO_PURE, 1>(a, s);
+    }
+    if (f64)
+        return sto ? launch_hmc_f64<QMC_RADIAL_STO, QMC_MAXCONF>(a, s)
+                   : launch_hmc_f64<QMC_RADIAL_GTO_PURE, QMC_MAXCONF>(a, s);
+    return sto ? launch_hmc_f32<QMC_RADIAL_STO, QMC_MAXCONF>(a, s)
+               : launch_hmc_f32<QMC_RADIAL_GTO_PURE, QMC_MAXCONF>(a, s);
+}
+}  // namespace
+
 extern "C" {
 
 const char* qmc_last_error(void) { return g_last_error.c_str(); }
@@ -274,6 +289,53 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
     }
     rc = precision == QMC_PREC_FP64 ? QMC_BY_RAD(launch_sweep_f64, a.t.radial, a, s)
                                     : QMC_BY_RAD(launch_sweep_f32, a.t.radial, a, s);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precision, QmcWalkers* wk, int64_t nw,
+                  const QmcHmcDesc* hm, qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!wk || !hm || !wk->pos || !wk->logabs || !wk->sign) return fail(QMC_ERR_ARG, "null walker state");
+    if (hm->traj_length < 1) return fail(QMC_ERR_ARG, "traj_length must be >= 1");
+    if (hm->keep && hm->keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
+    if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
+    if (nw == 0 || hm->nstep <= 0) return QMC_OK;
+    HArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.wk = *wk;
+    a.hm = *hm;
+    a.nw = nw;
+    a.drift = nullptr;
+    a.write_pos = 1;
+    hipStream_t s = (hipStream_t)stream;
+    rc = launch_hmc(a, precision, s);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision, const double* pos, int64_t nw,
+                 double* drift, double* logabs, double* sign, qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if (!pos || !drift) return fail(QMC_ERR_ARG, "null pos/drift");
+    if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
+    if (nw == 0) return QMC_OK;
+    HArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.wk = QmcWalkers{const_cast<double*>(pos), logabs, sign, nullptr, nullptr};  // positions are only read
+    a.hm = QmcHmcDesc{};
+    a.hm.traj_length = 1;
+    a.nw = nw;
+    a.drift = drift;
+    a.write_pos = 0;
+    hipStream_t s = (hipStream_t)stream;
+    rc = launch_hmc(a, precision, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;

@@ -464,4 +464,27 @@ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
 __device__ __forceinline__ double u01_open0(uint32_t v) { return ((double)v + 1.0) * 2.3283064365386963e-10; }
 __device__ __forceinline__ double u01_closed0(uint32_t v) { return (double)v * 2.3283064365386963e-10; }
 
+// Three standard normals from one Philox block, Box-Muller in FP64:
+// (x, y) -> sqrt(-2 ln u) (cos 2 pi v, sin 2 pi v), (z, w) -> sqrt(-2 ln u) cos 2 pi v.
+__device__ __forceinline__ void gauss3_f64(const u4 r, double& a, double& b, double& c) {
+    const double r1 = sqrt(-2.0 * log(u01_open0(r.x)));
+    const double r2 = sqrt(-2.0 * log(u01_open0(r.z)));
+    double s1, c1;
+    sincospi(2.0 * u01_closed0(r.y), &s1, &c1);
+    a = r1 * c1;
+    b = r1 * s1;
+    c = r2 * cospi(2.0 * u01_closed0(r.w));
+}
+
+// ------------------------------------------------------------ wave-local LDS
+// Orders one wavefront's LDS accesses across this point: waits for the
+// outstanding ones and keeps the compiler from moving accesses over it.  The
+// LDS serves a wavefront's requests in order, so stores issued before the call
+// are seen by loads of any lane of the same wavefront issued after it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 }  // namespace qmc

@@ -76,7 +76,7 @@ __device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs, i
 // read from the entry-ordered copy (row i = W row of the entry's AO).
 // WT is the storage type of W, AT the accumulator type; AO values may arrive in
 // either precision (FP64 for tight exponent groups in the mixed local energy).
-template <typename WT, int NM, bool DERIV, typename AT = WT>
+template <typename WT, int NM, bool DERIV, typename AT = WT, bool LAP = DERIV>
 struct MoSink {
     const WT* __restrict__ W;
     int nmo;
@@ -98,7 +98,7 @@ struct MoSink {
                 gx[m] += (AT)ax * w;
                 gy[m] += (AT)ay * w;
                 gz[m] += (AT)az * w;
-                lp[m] += (AT)al * w;
+                if (LAP) lp[m] += (AT)al * w;
             }
         }
     }
@@ -577,6 +577,324 @@ __global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restr
     const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
     if ((int64_t)L.wave * L.G >= a.nw) return;
     sweep_walkers<T, NM, RAD, AO>(a, L);
+}
+
+// ------------------------------------------------------------------ drift
+// ln J seen from one electron (jastrow_value's arithmetic) and the gradient
+// of ln J with respect to this electron, in the bulk precision T:
+// d/dr [w0 r / (1 + w r)] = w0 / (1 + w r)^2 (pade_jastrow_kernel.py of both
+// Jastrow families), the e-e pair weight w0 entering the gradient in full.
+template <typename T>
+__device__ __forceinline__ T jastrow_value_grad(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                                double y, double z, T (&g)[3]) {
+    T k = T(0);
+    g[0] = g[1] = g[2] = T(0);
+    if (t.use_ee) {
+        const T wee = (T)ldc(prm.w_ee, 0);
+        const T xs = (T)x, ys = (T)y, zs = (T)z;
+        for (int j0 = 0; j0 < t.nelec; j0 += 4) {
+            T xj[4], yj[4], zj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jl = L.wbase + (j0 + u < t.nelec ? j0 + u : 0);
+                xj[u] = __shfl(xs, jl);
+                yj[u] = __shfl(ys, jl);
+                zj[u] = __shfl(zs, jl);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                if (j < t.nelec) {
+                    const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
+                    const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+                    const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
+                    const T den = frcp(T(1) + wee * r);
+                    const bool other = j != L.e;
+                    k += other ? w0 * r * den : T(0);
+                    const T c = other ? T(2) * w0 * den * den * frcp(r) : T(0);
+                    g[0] += c * dx;
+                    g[1] += c * dy;
+                    g[2] += c * dz;
+                }
+            }
+        }
+    }
+    if (t.use_en) {
+        const T wen = (T)ldc(prm.w_en, 0);
+        for (int A = 0; A < t.natom; ++A) {
+            const T dx = (T)(x - t.jas_atoms[3 * A]), dy = (T)(y - t.jas_atoms[3 * A + 1]),
+                    dz = (T)(z - t.jas_atoms[3 * A + 2]);
+            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+            const T den = frcp(T(1) + wen * r);
+            k += r * den;
+            const T c = den * den * frcp(r);
+            g[0] += c * dx;
+            g[1] += c * dy;
+            g[2] += c * dz;
+        }
+    }
+    return k;
+}
+
+// log|psi| of the walker (every lane) and the drift grad_i ln|psi| of this
+// lane's electron (slater_jastrow.py:346-447 without the psi factor):
+//   grad_i ln|S| = sum_c omega_c sum_k (A_c^-1)[k, i] grad phi_{conf_c[k]}(r_i),
+//   omega_c = sum_{n using c for this spin} c_n D_n / S,
+// plus grad_i ln J.  A_c^-1 comes from the lane elimination of [A_c | I] (the
+// lane that pivoted column k holds row k); the rows are transposed through
+// the wavefront's LDS block lds[64][NM] so that lane i reads column i.  AO and
+// MO rows in T, elimination and combination in FP64.  MC bounds the unique
+// configurations per spin (1, or QMC_MAXCONF for multi-configuration plans):
+// the per-configuration values live in registers.
+template <typename T, int NM, int RAD, int MC, typename AO = AoGeneric>
+__device__ __forceinline__ double eval_drift(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                             double y, double z, double logc0, double* lds, double& sgn,
+                                             double (&dr)[3]) {
+    T mo[NM], gx[NM], gy[NM], gz[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = T(0);
+    // values and gradients only: the row's Laplacian terms are dead code here
+    MoSink<T, NM, true, T, false> sink{(const T*)prm.entry_w, t.nmo, mo, gx, gy, gz, nullptr};
+    AO::template row<T, RAD, true, false>(t, prm, x, y, z, sink);
+    T jg[3];
+    const T kj = jastrow_value_grad<T>(t, prm, L, x, y, z, jg);
+    const double logj = (double)seg_sum<T>(kj, L.wbase, t.nelec, t.nelec);
+
+    double ld[MC], sd[MC], s0[MC], s1[MC], s2[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        ld[c] = s0[c] = s1[c] = s2[c] = 0.0;
+        sd[c] = 1.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < t.nconf_max; ++c) {
+        double aa[NM], ib[NM];
+        gather_conf<NM>(t, L, c, mo, aa);
+#pragma unroll
+        for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
+        double lg, sg, tp;
+        int col;
+        lane_gauss_jordan<NM, true>(aa, ib, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        wave_lds_sync();  // the previous configuration's reads are done
+        if (L.lane_real && col >= 0) {
+#pragma unroll
+            for (int j = 0; j < NM; ++j) lds[(L.gbase + col) * NM + j] = ib[j];
+        }
+        wave_lds_sync();
+        double ga[NM], gb[NM], gc[NM];
+        gather_conf<NM>(t, L, c, gx, ga);
+        gather_conf<NM>(t, L, c, gy, gb);
+        gather_conf<NM>(t, L, c, gz, gc);
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            if (k < L.n_sig) {
+                const double inv = lds[(L.gbase + k) * NM + L.myrow];
+                v0 += inv * ga[k];
+                v1 += inv * gb[k];
+                v2 += inv * gc[k];
+            }
+        }
+#pragma unroll
+        for (int c2 = 0; c2 < MC; ++c2) {
+            if (c2 == c) {
+                ld[c2] = lg;
+                sd[c2] = sg;
+                s0[c2] = v0;
+                s1[c2] = v1;
+                s2[c2] = v2;
+            }
+        }
+    }
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    double logabs;
+    if (t.identity_conf) {
+        const double lu = __shfl(ld[0], up_lane), su = __shfl(sd[0], up_lane);
+        const double lo = __shfl(ld[0], dn_lane), so = __shfl(sd[0], dn_lane);
+        const double c0 = ldc(prm.ci, 0);
+        logabs = lu + (t.ndown > 0 ? lo : 0.0) + logc0;
+        sgn = su * (t.ndown > 0 ? so : 1.0) * (c0 < 0.0 ? -1.0 : 1.0);
+        dr[0] = s0[0];
+        dr[1] = s1[0];
+        dr[2] = s2[0];
+    } else {
+        double lu[MC], su[MC], lo[MC], so[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            lu[c] = __shfl(ld[c], up_lane);
+            su[c] = __shfl(sd[c], up_lane);
+            const double l_ = __shfl(ld[c], dn_lane), s_ = __shfl(sd[c], dn_lane);
+            lo[c] = t.ndown > 0 ? l_ : 0.0;
+            so[c] = t.ndown > 0 ? s_ : 1.0;
+        }
+        auto term = [&](int n, double& lg, double& sg) {
+            lg = 0.0;
+            sg = 1.0;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                if (c == t.ci_up[n]) {
+                    lg += lu[c];
+                    sg *= su[c];
+                }
+                if (c == t.ci_dn[n]) {
+                    lg += lo[c];
+                    sg *= so[c];
+                }
+            }
+        };
+        double lmax = -INFINITY;
+        for (int n = 0; n < t.nci; ++n) {
+            double lg, sg;
+            term(n, lg, sg);
+            lmax = fmax(lmax, lg);
+        }
+        if (!(lmax > -INFINITY)) lmax = 0.0;
+        double S = 0.0;
+        for (int n = 0; n < t.nci; ++n) {
+            double lg, sg;
+            term(n, lg, sg);
+            S += ldc(prm.ci, n) * sg * exp(lg - lmax);
+        }
+        logabs = lmax + log(fabs(S));
+        sgn = S < 0.0 ? -1.0 : 1.0;
+        dr[0] = dr[1] = dr[2] = 0.0;
+        for (int n = 0; n < t.nci; ++n) {
+            double lg, sg;
+            term(n, lg, sg);
+            const double wn = ldc(prm.ci, n) * sg * exp(lg - lmax) / S;
+            const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                if (c == mine) {
+                    dr[0] += wn * s0[c];
+                    dr[1] += wn * s1[c];
+                    dr[2] += wn * s2[c];
+                }
+            }
+        }
+    }
+    dr[0] += (double)jg[0];
+    dr[1] += (double)jg[1];
+    dr[2] += (double)jg[2];
+    return logabs + logj;
+}
+
+// Hamiltonian Monte Carlo steps with the walker state, momenta and the drift
+// at the current position in registers (sampler/hamiltonian.py:80-201).  With
+// nstep == 0 the kernel only evaluates log|psi|, sign and the drift at the
+// given positions (write_pos == 0 then leaves the positions untouched).
+struct HArgs {
+    Tables t;
+    QmcParams p;
+    QmcWalkers wk;
+    QmcHmcDesc hm;
+    int64_t nw;
+    double* drift;  // [nw][nelec][3] at the final positions, or null
+    int write_pos;
+};
+
+template <typename T, int NM, int RAD, int MC, typename AO = AoGeneric>
+__global__ __launch_bounds__(kBlock) void hmc_kernel(HArgs a) {
+    __shared__ double lds_block[kBlock * NM];
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    double* lds = lds_block + (threadIdx.x >> 6) * 64 * NM;
+    const Tables& t = a.t;
+    const int Ne = t.nelec;
+    double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
+    double x = pp[0], y = pp[1], z = pp[2];
+    const double logc0 = log(fabs(ldc(a.p.ci, 0)));
+    const uint64_t wid = a.wk.walker_id ? (uint64_t)a.wk.walker_id[L.wclamp] : (uint64_t)L.wclamp;
+    const uint32_t k0 = (uint32_t)a.hm.seed, k1 = (uint32_t)(a.hm.seed >> 32);
+    const double eps = a.hm.step_size;
+    const int nstep = a.hm.nstep, traj = a.hm.traj_length;
+    // accepted state (position, log|psi|, sign, drift) and the trajectory point
+    double lp = 0.0, sgn = 1.0, g[3] = {0.0, 0.0, 0.0};
+    double qx = x, qy = y, qz = z, px = 0.0, py = 0.0, pz = 0.0, e0 = 0.0;
+    int nacc = 0;
+    // One evaluation site (the kernel holds a single copy of the wavefunction
+    // code): evaluation 0 is the starting point, then traj per step.  s and l
+    // are uniform over the wavefront.
+    int s = -1, l = 0;
+    for (;;) {
+        double sgn_n, gn[3];
+        const double lpn = eval_drift<T, NM, RAD, MC, AO>(t, a.p, L, qx, qy, qz, logc0, lds, sgn_n, gn);
+        if (s >= 0) {
+            // momentum update at the new point: full step, or the closing half step
+            const double f = (l + 1 < traj) ? 2.0 * eps : eps;  // p -= f/2 grad U, grad U = -2 drift
+            px += f * gn[0];
+            py += f * gn[1];
+            pz += f * gn[2];
+            if (++l < traj) {
+                qx += eps * px;
+                qy += eps * py;
+                qz += eps * pz;
+                continue;
+            }
+            const uint64_t step = (uint64_t)(a.hm.step0 + s);
+            const double e1 = -2.0 * lpn + 0.5 * seg_sum(px * px + py * py + pz * pz, L.wbase, Ne, Ne);
+            const u4 ua =
+                philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u}, k0, k1);
+            if (exp(e0 - e1) >= u01_closed0(ua.x)) {
+                x = qx;
+                y = qy;
+                z = qz;
+                lp = lpn;
+                sgn = sgn_n;
+                g[0] = gn[0];
+                g[1] = gn[1];
+                g[2] = gn[2];
+                nacc++;
+            }
+            if (a.hm.keep && L.valid && (int64_t)step >= a.hm.keep_from &&
+                ((int64_t)step - a.hm.keep_from) % a.hm.keep_every == 0) {
+                const int64_t ik = ((int64_t)step - a.hm.keep_from) / a.hm.keep_every;
+                double* kp = a.hm.keep + ((size_t)ik * a.nw + L.w) * 3 * Ne + 3 * L.e;
+                kp[0] = x;
+                kp[1] = y;
+                kp[2] = z;
+            }
+        } else {
+            lp = lpn;
+            sgn = sgn_n;
+            g[0] = gn[0];
+            g[1] = gn[1];
+            g[2] = gn[2];
+        }
+        if (++s >= nstep) break;
+        // next step: momenta p ~ N(0, 1), E0 = U + |p|^2 / 2 with U = -2 ln|psi|,
+        // the opening half step in momentum and the first position update
+        const uint64_t step = (uint64_t)(a.hm.step0 + s);
+        const u4 rnd = philox4x32_10(
+            u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, (uint32_t)L.e | 0x40000000u}, k0, k1);
+        gauss3_f64(rnd, px, py, pz);
+        e0 = -2.0 * lp + 0.5 * seg_sum(px * px + py * py + pz * pz, L.wbase, Ne, Ne);
+        px += eps * g[0];
+        py += eps * g[1];
+        pz += eps * g[2];
+        qx = x + eps * px;
+        qy = y + eps * py;
+        qz = z + eps * pz;
+        l = 0;
+    }
+    if (L.valid) {
+        if (a.write_pos) {
+            pp[0] = x;
+            pp[1] = y;
+            pp[2] = z;
+        }
+        if (a.drift) {
+            double* o = a.drift + ((size_t)L.w * Ne + L.e) * 3;
+            o[0] = g[0];
+            o[1] = g[1];
+            o[2] = g[2];
+        }
+        if (L.e == 0) {
+            if (a.wk.logabs) a.wk.logabs[L.w] = lp;
+            if (a.wk.sign) a.wk.sign[L.w] = sgn;
+            if (a.wk.naccept) a.wk.naccept[L.w] += nacc;
+        }
+    }
 }
 
 // AO values and derivatives in the reference layout (atomic_orbitals.py:578-609).

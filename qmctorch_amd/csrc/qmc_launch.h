@@ -47,6 +47,9 @@ template <int RAD> int launch_sweep_f64(const SArgs& a, hipStream_t s);
 template <int RAD> int launch_grad(const GArgs& a, hipStream_t s);
 template <int RAD> int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
 template <int RAD> int launch_eloc_grad(const EArgs& a, hipStream_t s);
+// HMC / drift: MC = 1 for single-configuration plans, QMC_MAXCONF otherwise
+template <int RAD, int MC> int launch_hmc_f32(const HArgs& a, hipStream_t s);
+template <int RAD, int MC> int launch_hmc_f64(const HArgs& a, hipStream_t s);
 // batched launches: jobs (device) sorted into one register bucket, nwaves in total
 template <int RAD> int launch_sweep_batch_f32(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s);
 template <int RAD> int launch_sweep_batch_f64(const SJob* jobs, int njobs, int nwaves, int bucket, hipStream_t s);

@@ -268,13 +268,28 @@ class SlaterJastrow(WaveFunction):
                   en.jastrow_kernel.weight if en is not None else None]
         return _ELocFunction.apply(self, pos, *params)
 
+    def drift(self, pos: torch.Tensor, precision: Optional[str] = None):
+        """(grad_i ln|psi| as (Nbatch, Nelec, 3), log|psi|, sign(psi)) in float64
+        from qmc_wf_drift; ``precision`` defaults to the sampling precision."""
+        precision = precision or self.precision
+        x = self._check_pos(pos)
+        nw = x.shape[0]
+        plan = self.device_plan()
+        prm, keep = self._params(precision)
+        dr = torch.empty(nw, self.nelec, 3, dtype=torch.float64, device=x.device)
+        la = torch.empty(nw, dtype=torch.float64, device=x.device)
+        sg = torch.empty(nw, dtype=torch.float64, device=x.device)
+        N.check(N.load().qmc_wf_drift(plan.handle, prm, _PREC[precision], x.data_ptr(), nw, dr.data_ptr(),
+                                      la.data_ptr(), sg.data_ptr(), N.current_stream()))
+        return dr, la, sg
+
     def gradients_jacobi(self, x: torch.Tensor, sum_grad: Optional[bool] = False,
                          pdf: Optional[bool] = False) -> torch.Tensor:
         """grad psi (or grad psi^2 with pdf=True), shape (Nbatch, 3 Nelec)
-        (slater_jastrow.py:346-447): psi * (grad ln J + grad S / S)."""
-        rows = eloc_adjoint_rows(self, x, need_rows=False)
-        psi = self._eval(x, N.QMC_EVAL_PSI, "fp64")["psi"]
-        out = rows["drift"].view(x.shape[0], -1) * psi[:, None]
+        (slater_jastrow.py:346-447): psi * (grad ln J + grad S / S), FP64."""
+        dr, la, sg = self.drift(x, "fp64")
+        psi = sg * torch.exp(la)
+        out = dr.view(x.shape[0], -1) * psi[:, None]
         if pdf:
             out = 2.0 * out * psi[:, None]
         return out.to(torch.get_default_dtype())

@@ -47,6 +47,7 @@ def test_ctypes_struct_layout_matches_header(tmp_path):
         "QmcWfOut": N.QmcWfOut,
         "QmcWalkers": N.QmcWalkers,
         "QmcSweepDesc": N.QmcSweepDesc,
+        "QmcHmcDesc": N.QmcHmcDesc,
         "QmcSweepJob": N.QmcSweepJob,
         "QmcEvalJob": N.QmcEvalJob,
     }
