@@ -126,6 +126,76 @@ def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=200, warm=3):
                       % (nw, sample_steps, dt)}
 
 
+def hmc_leg(mol, wf, nw, steps=200, L=10, eps=0.05, ref_steps=3, reference=True):
+    """Hamiltonian Monte Carlo (qmc_hmc_sweep) on the same system and walkers:
+    HMC steps/s and drift evaluations/s (L per step), acceptance, and the
+    reference algorithm (autograd leapfrog, hamiltonian.py:143-201) on
+    torch-ROCm FP64 on the same GPU."""
+    import numpy as np
+    import torch
+
+    from qmctorch_amd import _native as N
+    from tests import helpers
+
+    dev = torch.device("cuda")
+    x = helpers.positions(mol, nw, seed=4).to(dev).contiguous()
+    logabs = torch.empty(nw, dtype=torch.float64, device=dev)
+    sign = torch.empty_like(logabs)
+    nacc = torch.zeros(nw, dtype=torch.int32, device=dev)
+    wk = N.QmcWalkers()
+    wk.pos, wk.logabs, wk.sign, wk.naccept, wk.walker_id = x.data_ptr(), logabs.data_ptr(), sign.data_ptr(), \
+        nacc.data_ptr(), 0
+    hm = N.QmcHmcDesc()
+    hm.traj_length, hm.step_size, hm.seed, hm.keep, hm.keep_every = L, eps, 2024, 0, 1
+    plan = wf.device_plan()
+    prm, keep = wf._params(wf.precision)
+    prec = N.QMC_PREC_MIXED if wf.precision == "mixed" else N.QMC_PREC_FP64
+    lib, stream = N.load(), N.current_stream()
+    per_launch = 10
+
+    def run(step0, nstep, events=None):
+        for s0 in range(step0, step0 + nstep, per_launch):
+            hm.step0, hm.nstep = s0, min(per_launch, step0 + nstep - s0)
+            if events is not None:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+            N.check(lib.qmc_hmc_sweep(plan.handle, prm, prec, wk, nw, hm, stream))
+            if events is not None:
+                e1.record()
+                events.append((e0, e1))
+
+    run(0, 20)
+    torch.cuda.synchronize()
+    nacc.zero_()
+    events = []
+    t0 = time.perf_counter()
+    run(20, steps, events)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    launch_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
+    out = {"walkers": nw, "traj_length": L, "step_size": eps, "steps_per_launch": per_launch,
+           "hmc_steps_per_s": nw * steps / dt, "drift_evals_per_s": nw * steps * L / dt,
+           "launch_ms": launch_ms, "acceptance": float(nacc.sum().item()) / (nw * steps),
+           "precision": wf.precision}
+    if not reference:
+        return out
+    # baseline: the reference algorithm on torch-ROCm (FP64), same walkers, a few steps
+    O, orc = _restatement(mol, wf)
+    orc.to_device("cuda")
+    pos = helpers.positions(mol, nw, seed=4).cuda()
+    O.hamiltonian(orc.pdf, pos.clone(), 1, step_size=eps, L=L, seed=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    O.hamiltonian(orc.pdf, pos.clone(), ref_steps, step_size=eps, L=L, seed=0)
+    torch.cuda.synchronize()
+    rdt = time.perf_counter() - t0
+    out["reference_gpu_torch"] = {"hmc_steps_per_s": nw * ref_steps / rdt,
+                                  "sample": "%d walkers x %d HMC steps (L=%d), autograd leapfrog, torch-ROCm "
+                                            "float64 (%.1f s)" % (nw, ref_steps, L, rdt)}
+    out["speedup_vs_reference_gpu_torch"] = out["hmc_steps_per_s"] / out["reference_gpu_torch"]["hmc_steps_per_s"]
+    return out
+
+
 def sweep_precision_effect(mol, wf, nw, steps=2000, seed=77):
     """Effect of the FP32-bulk sweep on the sampled energy: the same chains
     (same start, seed and walker ids) run by the FP32-bulk and the FP64 sweep;
@@ -322,6 +392,7 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the H2 / LiH / H2O-optimisation legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-system batched-launch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-loop leg")
+    ap.add_argument("--no-hmc", action="store_true", help="skip the Hamiltonian Monte Carlo leg")
     ap.add_argument("--serving-rates", default="50,500,2000", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -466,6 +537,8 @@ def main():
             "flops_per_walker_step": F + 4 * T,
         },
     }
+    if not args.no_hmc:
+        res["hmc"] = hmc_leg(mol, wf, nw, reference=not args.no_cpu_baseline)
     if not args.no_configs:
         res["configs"] = config_legs(args.precision)
     if not args.no_batched:

@@ -779,8 +779,9 @@ def hmc_step(pdf, q_init, p, eps, L, u):
 
 def hamiltonian(pdf, pos, nstep, step_size=0.2, L=10, ntherm=-1, ndecor=1, seed=None):
     """Hamiltonian.__call__ (hamiltonian.py:80-141): momenta from torch.randn,
-    uniforms from torch.rand, in that order every step.  Returns (positions
-    kept, acceptance rate)."""
+    uniforms from torch.rand, in that order every step, drawn on the device of
+    ``pos`` (the reference draws them on the host, which confines it to CPU
+    tensors).  Returns (positions kept, acceptance rate)."""
     if seed is not None:
         torch.manual_seed(seed)
     if ntherm < 0:
@@ -788,8 +789,8 @@ def hamiltonian(pdf, pos, nstep, step_size=0.2, L=10, ntherm=-1, ndecor=1, seed=
     q = pos.detach().clone()
     kept, rate, idecor = [], 0.0, 0
     for istep in range(nstep):
-        p = torch.randn(q.shape, dtype=q.dtype)
-        u = torch.rand(q.shape[0], dtype=q.dtype)
+        p = torch.randn(q.shape, dtype=q.dtype, device=q.device)
+        u = torch.rand(q.shape[0], dtype=q.dtype, device=q.device)
         q, acc = hmc_step(pdf, q, p, step_size, L, u)
         rate += float(acc.double().mean())
         if istep >= ntherm:

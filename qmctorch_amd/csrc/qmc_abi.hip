@@ -312,6 +312,10 @@ int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precisio
     a.drift = nullptr;
     a.write_pos = 1;
     hipStream_t s = (hipStream_t)stream;
+    if (plan->spec && plan->use_spec && plan->spec->hmc(a, precision, s) == 0) {
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     rc = launch_hmc(a, precision, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());
@@ -335,6 +339,10 @@ int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision
     a.drift = drift;
     a.write_pos = 0;
     hipStream_t s = (hipStream_t)stream;
+    if (plan->spec && plan->use_spec && plan->spec->hmc(a, precision, s) == 0) {
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     rc = launch_hmc(a, precision, s);
     if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
     QMC_HIP(hipGetLastError());

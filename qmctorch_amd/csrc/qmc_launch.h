@@ -71,6 +71,7 @@ struct SpecEntry {
     int (*sweep_batch)(const SJob* jobs, int njobs, int nwaves, int precision, hipStream_t s);
     int (*wf)(const KArgs& a, int precision, int mode, hipStream_t s);  // FP64 local energy
     int (*wf_batch)(const WJob* jobs, int njobs, int nwaves, int precision, int mode, hipStream_t s);
+    int (*hmc)(const HArgs& a, int precision, hipStream_t s);  // mixed HMC / drift
 };
 
 // the generated layout with this signature and bucket, or nullptr

@@ -7,7 +7,7 @@ waits for.  For the layouts of the benchmark molecules this module writes a
 row in which every one of those values, and the contraction coefficients,
 are compile-time constants (only exponents, MO rows and positions are still
 read, at constant offsets the compiler can issue together), and instantiates the sweep, the
-batched sweep and the (FP64) local-energy kernels with it.  At plan creation
+batched sweep, the (FP64) local-energy and the (mixed) HMC kernels with it.  At plan creation
 the library matches a plan to a generated layout by the signature computed
 here and in qmc_abi.hip over the same fields; other plans use the generic row.
 
@@ -19,6 +19,7 @@ from typing import Dict, List, Tuple
 
 import numpy as np
 
+QMC_MAXCONF = 8  # unique configurations per spin handled by the kernels (include/qmc_mi355x.h)
 FNV_OFFSET = 0xCBF29CE484222325
 FNV_PRIME = 0x100000001B3
 MASK = (1 << 64) - 1
@@ -151,6 +152,18 @@ int spec_%(name)s_sweep_batch(const SJob* jobs, int njobs, int nwaves, int prec,
 }  // namespace qmc
 '''
 
+HMC = '''
+constexpr int kMC = %(mc)d;
+
+int spec_%(name)s_hmc(const HArgs& a, int prec, hipStream_t s) {
+    if (prec != QMC_PREC_MIXED || a.t.nconf_max > kMC) return -1;
+    hipLaunchKernelGGL((hmc_kernel<float, kNM, kRAD, kMC, AoSpec>), grid_for(a.t, a.nw), dim3(kBlock), 0, s, a);
+    return 0;
+}
+
+}  // namespace qmc
+'''
+
 ELOC = '''
 int spec_%(name)s_wf(const KArgs& a, int prec, int mode, hipStream_t s) {
     if (prec != QMC_PREC_FP64 || mode != QMC_EVAL_ELOC) return -1;
@@ -178,7 +191,7 @@ namespace qmc {
 #if !defined(__HIP_DEVICE_COMPILE__)
 const SpecEntry* find_spec(uint64_t signature, int nm) {
     static const SpecEntry table[] = {
-%(rows)s        {0, 0, nullptr, nullptr, nullptr, nullptr, nullptr}};
+%(rows)s        {0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
     for (const SpecEntry* e = table; e->name; ++e)
         if (e->signature == signature && e->nm == nm) return e;
     return nullptr;
@@ -189,28 +202,33 @@ const SpecEntry* find_spec(uint64_t signature, int nm) {
 '''
 
 
-def unit_sources(name: str, tab, nm: int) -> Dict[str, str]:
-    """Two units per layout (sweeps; local energy) so that they compile in parallel."""
+def unit_sources(name: str, tab, nm: int, mc: int = 1) -> Dict[str, str]:
+    """Three units per layout (sweeps; local energy; HMC) so that they compile
+    in parallel.  mc bounds the unique configurations per spin of the HMC
+    instance (1, or QMC_MAXCONF)."""
     head = ROW % dict(name=name, ngroup=len(tab.grp_atom), nentry=len(tab.gao_mu), nm=nm, rad=int(tab.radial),
                       body=row_source(tab))
     return {"qmc_spec_%s_sweep.hip" % name: head + SWEEP % dict(name=name),
-            "qmc_spec_%s_eloc.hip" % name: head + ELOC % dict(name=name)}
+            "qmc_spec_%s_eloc.hip" % name: head + ELOC % dict(name=name),
+            "qmc_spec_%s_hmc.hip" % name: head + HMC % dict(name=name, mc=mc)}
 
 
 def registry_source(layouts) -> str:
     decls, rows = "", ""
-    for name, tab, nm in layouts:
+    for name, tab, nm, _ in layouts:
         decls += ("int spec_%(n)s_sweep(const SArgs&, int, hipStream_t);\n"
                   "int spec_%(n)s_sweep_batch(const SJob*, int, int, int, hipStream_t);\n"
                   "int spec_%(n)s_wf(const KArgs&, int, int, hipStream_t);\n"
-                  "int spec_%(n)s_wf_batch(const WJob*, int, int, int, int, hipStream_t);\n" % dict(n=name))
+                  "int spec_%(n)s_wf_batch(const WJob*, int, int, int, int, hipStream_t);\n"
+                  "int spec_%(n)s_hmc(const HArgs&, int, hipStream_t);\n" % dict(n=name))
         rows += ("        {0x%016xull, %d, \"%s\", &spec_%s_sweep, &spec_%s_sweep_batch, &spec_%s_wf, "
-                 "&spec_%s_wf_batch},\n" % (layout_signature(tab), nm, name, name, name, name, name))
+                 "&spec_%s_wf_batch, &spec_%s_hmc},\n" % (layout_signature(tab), nm, name, name, name, name, name,
+                                                           name))
     return REGISTRY % dict(decls=decls, rows=rows)
 
 
-def catalog_layouts() -> List[Tuple[str, object, int]]:
-    """(name, SystemTables, register bucket) of the benchmark molecules:
+def catalog_layouts() -> List[Tuple[str, object, int, int]]:
+    """(name, SystemTables, register bucket, HMC configuration bound) of the benchmark molecules:
     ground states of H2/STO-3G, LiH/6-31G, H2O, N2, C6H6/cc-pVDZ and
     LiH/6-31G CAS(2,2)."""
     from ..serving import catalog
@@ -226,7 +244,8 @@ def catalog_layouts() -> List[Tuple[str, object, int]]:
         nmo = int(max(cu.max(initial=-1), cd.max(initial=-1))) + 1
         tab = SystemTables(mol.basis, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown, (cu, cd), nmo,
                            True, True, atomic_orbital_norm(mol.basis), np.asarray(mol.basis.bas_exp))
-        out.append((name, tab, nm_bucket_fine(nmo, mol.nup, mol.ndown)))
+        nconf = max(len({tuple(r) for r in cu.tolist()}), len({tuple(r) for r in cd.tolist()}))
+        out.append((name, tab, nm_bucket_fine(nmo, mol.nup, mol.ndown), 1 if nconf == 1 else QMC_MAXCONF))
     return out
 
 
@@ -236,8 +255,8 @@ def write_sources(outdir: str, layouts=None) -> List[str]:
     os.makedirs(outdir, exist_ok=True)
     layouts = catalog_layouts() if layouts is None else layouts
     files: Dict[str, str] = {}
-    for name, tab, nm in layouts:
-        for fname, text in unit_sources(name, tab, nm).items():
+    for name, tab, nm, mc in layouts:
+        for fname, text in unit_sources(name, tab, nm, mc).items():
             files[os.path.join(outdir, fname)] = text
     files[os.path.join(outdir, "qmc_spec_registry.hip")] = registry_source(layouts)
     for path, text in files.items():

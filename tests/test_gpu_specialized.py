@@ -3,7 +3,9 @@
 Every catalog molecule's plan must find its generated layout (the signature
 computed by specialize.py equals the one computed by the library), and the
 generated kernels must reproduce the table-driven ones: the FP64 local energy
-within 1e-13 relative and the mixed-precision sweep walker for walker.
+within 1e-13 relative, the mixed-precision sweep walker for walker, and the
+mixed-precision drift of the HMC kernel within FP32 rounding (1e-4 relative
+to 1 + |drift|, median below 1e-6).
 """
 import math
 import sys
@@ -82,3 +84,24 @@ def test_generated_layout_matches_table_driven(system, configs, name):
          % (rel, same, int(ns.sum()), int(ng.sum())))
     assert rel < 1e-13
     assert same >= 0.99
+
+
+@pytest.mark.parametrize("system,configs,name", LAYOUTS)
+def test_generated_hmc_drift_matches_table_driven(system, configs, name):
+    _say("case generated hmc", name)
+    mol, wf = _wf(system, configs)
+    plan = wf.device_plan()
+    assert plan.specialization == name
+    pos = catalog.initial_positions(mol, 300, seed=8)
+    ds, ls, ss = wf.drift(pos, "mixed")
+    plan.set_specialized(False)
+    try:
+        dg, lg, sg = wf.drift(pos, "mixed")
+    finally:
+        plan.set_specialized(True)
+    rel = ((ds - dg).norm(dim=-1) / (1.0 + dg.norm(dim=-1))).view(-1)
+    _say("  drift rel diff median %.2e max %.2e, log|psi| max diff %.2e"
+         % (float(rel.median()), float(rel.max()), float((ls - lg).abs().max())))
+    assert float(rel.median()) < 1e-6 and float(rel.max()) < 1e-4
+    assert float((ls - lg).abs().max()) < 1e-4
+    assert torch.equal(ss, sg)
