@@ -18,8 +18,10 @@ namespace qmc {
 // kernel argument or from a job table in device memory.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define QMC_CONST __attribute__((address_space(4)))
+#define QMC_LDS __attribute__((address_space(3)))
 #else
 #define QMC_CONST
+#define QMC_LDS
 #endif
 
 template <typename X>

@@ -76,7 +76,9 @@ __device__ __forceinline__ int find_job(const J* __restrict__ jobs, int njobs, i
 // read from the entry-ordered copy (row i = W row of the entry's AO).
 // WT is the storage type of W, AT the accumulator type; AO values may arrive in
 // either precision (FP64 for tight exponent groups in the mixed local energy).
-template <typename WT, int NM, bool DERIV, typename AT = WT, bool LAP = DERIV>
+// WLDS: W is the block's copy in LDS (qmc_dyn_lds) instead of the plan's
+// constant-space table.
+template <typename WT, int NM, bool DERIV, typename AT = WT, bool LAP = DERIV, bool WLDS = false>
 struct MoSink {
     const WT* __restrict__ W;
     int nmo;
@@ -85,11 +87,24 @@ struct MoSink {
     AT* gy;
     AT* gz;
     AT* lp;
+    // Generated AO rows call fence() at every exponent group.  With W in LDS it
+    // replaces W by an opaque copy that depends on the MO accumulators, so the
+    // next group's W reads are issued when that group is reached (not all at the
+    // top of the evaluation, which would hold every W value in a VGPR, nor once
+    // before the step loop).  Without LDS it does nothing.
+    __device__ __forceinline__ void fence() {
+        if constexpr (WLDS) {
+            const QMC_LDS WT* w = (const QMC_LDS WT*)W;
+            asm volatile("" : "+s"(w) : "v"(mo[0]));
+            W = (const WT*)w;
+        }
+    }
     // row i of the entry-ordered W holds NM columns (zeros past nmo), so the
     // accumulation needs no per-column test
     template <typename V>
     __device__ __forceinline__ void operator()(int i, int, V ao, V ax, V ay, V az, V al) const {
-        const QMC_CONST WT* wr = (const QMC_CONST WT*)W + (size_t)i * NM;  // W row of AO entry i
+        using WP = typename std::conditional<WLDS, const QMC_LDS WT*, const QMC_CONST WT*>::type;
+        const WP wr = (WP)W + (size_t)i * NM;  // W row of AO entry i
 #pragma unroll
         for (int m = 0; m < NM; ++m) {
             const AT w = (AT)wr[m];
@@ -445,13 +460,19 @@ struct SArgs {
     int64_t nw;
 };
 
-template <typename T, int NM, int RAD, typename AO = AoGeneric>
+// Dynamic LDS of the kernels that stage parameter rows per block.
+extern __shared__ __attribute__((aligned(16))) unsigned char qmc_dyn_lds[];
+
+// wlds: with WLDS, this wavefront's LDS copy of the entry-ordered W rows
+template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false>
 __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
-                                              double y, double z, double logc0, double& sgn) {
+                                              double y, double z, double logc0, double& sgn,
+                                              const T* wlds = nullptr) {
     T mo[NM];
 #pragma unroll
     for (int m = 0; m < NM; ++m) mo[m] = T(0);
-    MoSink<T, NM, false> sink{(const T*)prm.entry_w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
+    const T* w = WLDS ? wlds : (const T*)prm.entry_w;
+    MoSink<T, NM, false, T, false, WLDS> sink{w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
     AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
     const T kj = jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
@@ -467,8 +488,8 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
 // state in registers (sampler/metropolis.py:134-165, 256-298).
-template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L) {
+template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false>
+__device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L, const T* wlds = nullptr) {
     const Tables& t = a.t;
     const int Ne = t.nelec;
     double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
@@ -477,7 +498,7 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     double lp;
     const double logc0 = log(fabs(ldc(a.p.ci, 0)));  // single-configuration CI weight, once per launch
     if (a.sw.init_logpsi) {
-        lp = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, x, y, z, logc0, sgn);
+        lp = eval_logpsi<T, NM, RAD, AO, WLDS>(t, a.p, L, x, y, z, logc0, sgn, wlds);
     } else {
         lp = a.wk.logabs[L.wclamp];
         sgn = a.wk.sign[L.wclamp];
@@ -517,7 +538,7 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        double lpn = eval_logpsi<T, NM, RAD, AO>(t, a.p, L, xn, yn, zn, logc0, sgn_n);
+        double lpn = eval_logpsi<T, NM, RAD, AO, WLDS>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
         if (!(lpn > lp_floor)) lpn = lp_floor;
         // acceptance uniform: counter slot 0x80000000 of the same walker and step;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
@@ -560,6 +581,27 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
     sweep_walkers<T, NM, RAD, AO>(a, L);
 }
 
+// sweep_kernel with the entry-ordered W rows staged in LDS once per block
+// (dynamic LDS of nentry * NM * sizeof(T) bytes, set by the launcher), for
+// generated AO rows (whose fence() calls pace the W reads).  W values then
+// reach the MO accumulation by LDS broadcast reads instead of scalar loads,
+// which a 50-step loop otherwise hoists and spills to VGPR lanes.  Two waves
+// per SIMD are requested explicitly: the extra VGPRs of the LDS operands would
+// otherwise leave one.
+template <typename T, int NM, int RAD, typename AO>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void sweep_kernel_wlds(SArgs a) {
+    {
+        T* w = (T*)qmc_dyn_lds;
+        const T* src = (const T*)a.p.entry_w;
+        const int n = a.t.nentry * NM;
+        for (int i = threadIdx.x; i < n; i += kBlock) w[i] = src[i];
+        __syncthreads();  // before any wavefront of the block leaves
+    }
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD, AO, true>(a, L, (const T*)qmc_dyn_lds);
+}
+
 // Sweeps of many systems (each with its own walkers, step count and seed) in
 // one launch; job j owns global waves [wave0[j], wave0[j+1]).  The arithmetic
 // of a walker does not depend on the other jobs of the launch.
@@ -577,6 +619,26 @@ __global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restr
     const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
     if ((int64_t)L.wave * L.G >= a.nw) return;
     sweep_walkers<T, NM, RAD, AO>(a, L);
+}
+
+// sweep_batch_kernel with W in LDS: every wavefront stages the W rows of its
+// own job into its slice of wstride elements (dynamic LDS of 4 slices), so
+// each walker runs the arithmetic of sweep_kernel_wlds exactly.
+template <typename T, int NM, int RAD, typename AO>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void sweep_batch_kernel_wlds(
+    const SJob* __restrict__ jobs, int njobs, int nwaves, int wstride) {
+    const int wave = global_wave();
+    if (wave >= nwaves) return;
+    const int j = find_job(jobs, njobs, wave);
+    const SArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    T* w = (T*)qmc_dyn_lds + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * wstride;  // wave-uniform
+    const T* src = (const T*)a.p.entry_w;
+    const int n = a.t.nentry * NM;  // <= wstride (one layout per launch)
+    for (int i = L.lane; i < n; i += 64) w[i] = src[i];
+    wave_lds_sync();
+    sweep_walkers<T, NM, RAD, AO, true>(a, L, w);
 }
 
 // ------------------------------------------------------------------ drift

@@ -96,6 +96,7 @@ def row_source(tab) -> str:
                     out.append("}")
                 return out
 
+            lines.append("            sink.fence();")
             lines.append("            // group %d: %d primitives from %d, entries %d..%d%s"
                          % (g, npr, p0, i0, i0 + nao - 1, ", FP64 in the mixed local energy" if tight else ""))
             if tight:
@@ -138,14 +139,13 @@ constexpr int kRAD = %(rad)d;
 SWEEP = '''
 int spec_%(name)s_sweep(const SArgs& a, int prec, hipStream_t s) {
     if (prec != QMC_PREC_MIXED) return -1;
-    hipLaunchKernelGGL((sweep_kernel<float, kNM, kRAD, AoSpec>), grid_for(a.t, a.nw), dim3(kBlock), 0, s, a);
+%(launch)s
     return 0;
 }
 
 int spec_%(name)s_sweep_batch(const SJob* jobs, int njobs, int nwaves, int prec, hipStream_t s) {
     if (prec != QMC_PREC_MIXED) return -1;
-    hipLaunchKernelGGL((sweep_batch_kernel<float, kNM, kRAD, AoSpec>), grid_for_waves(nwaves), dim3(kBlock), 0, s,
-                       jobs, njobs, nwaves);
+%(launch_batch)s
     return 0;
 }
 
@@ -202,13 +202,37 @@ const SpecEntry* find_spec(uint64_t signature, int nm) {
 '''
 
 
+# W rows of at most this many bytes are staged in LDS by the single-system sweep
+WLDS_MAX_BYTES = 8192
+
+
+def sweep_launch(nentry: int, nm: int) -> Tuple[str, str]:
+    """Launches of the single-system and batched sweeps: both with the W rows
+    staged in LDS when they fit (the two must run the same arithmetic, so that
+    a walker's chain is the same alone and in a batch), else both without."""
+    if nentry * nm * 4 <= WLDS_MAX_BYTES:
+        single = ("    hipLaunchKernelGGL((sweep_kernel_wlds<float, kNM, kRAD, AoSpec>), grid_for(a.t, a.nw), dim3(kBlock),\n"
+                  "                       (size_t)a.t.nentry * kNM * sizeof(float), s, a);")
+        batch = ("    hipLaunchKernelGGL((sweep_batch_kernel_wlds<float, kNM, kRAD, AoSpec>), grid_for_waves(nwaves),\n"
+                 "                       dim3(kBlock), (kBlock / 64) * %d * sizeof(float), s, jobs, njobs, nwaves, %d);"
+                 % (nentry * nm, nentry * nm))
+        return single, batch
+    single = ("    hipLaunchKernelGGL((sweep_kernel<float, kNM, kRAD, AoSpec>), grid_for(a.t, a.nw), dim3(kBlock), 0, s, "
+              "a);")
+    batch = ("    hipLaunchKernelGGL((sweep_batch_kernel<float, kNM, kRAD, AoSpec>), grid_for_waves(nwaves), dim3(kBlock), "
+             "0, s,\n                       jobs, njobs, nwaves);")
+    return single, batch
+
+
 def unit_sources(name: str, tab, nm: int, mc: int = 1) -> Dict[str, str]:
     """Three units per layout (sweeps; local energy; HMC) so that they compile
     in parallel.  mc bounds the unique configurations per spin of the HMC
     instance (1, or QMC_MAXCONF)."""
     head = ROW % dict(name=name, ngroup=len(tab.grp_atom), nentry=len(tab.gao_mu), nm=nm, rad=int(tab.radial),
                       body=row_source(tab))
-    return {"qmc_spec_%s_sweep.hip" % name: head + SWEEP % dict(name=name),
+    single, batch = sweep_launch(len(tab.gao_mu), nm)
+    sweep = SWEEP % dict(name=name, launch=single, launch_batch=batch)
+    return {"qmc_spec_%s_sweep.hip" % name: head + sweep,
             "qmc_spec_%s_eloc.hip" % name: head + ELOC % dict(name=name),
             "qmc_spec_%s_hmc.hip" % name: head + HMC % dict(name=name, mc=mc)}
 
