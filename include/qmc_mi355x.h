@@ -139,8 +139,10 @@ typedef struct QmcWalkers {
  * with covariance sigma2 * I per electron (metropolis.py:206-212: sigma2 =
  * step/(2 sqrt(2 ln 2))), acceptance min(1, psi'^2/psi^2) >= U.  Steps
  * step0 .. step0+nstep-1 of the chain are performed; random numbers come from
- * Philox4x32-10 keyed by seed and counted by (walker id, step, electron), so a
- * walker's trajectory does not depend on how walkers are batched.  Positions
+ * Philox4x32-10 keyed by seed and counted by (walker id, step, electron), one
+ * block per electron and step (the acceptance uniform is made of the bits of
+ * electron 0's block that its proposal leaves unused), so a walker's
+ * trajectory does not depend on how walkers are batched.  Positions
  * of steps s with s >= keep_from and (s - keep_from) % keep_every == 0 are
  * written to keep[(s - keep_from)/keep_every][walker] when keep != NULL.
  */

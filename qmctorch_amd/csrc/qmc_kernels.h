@@ -536,14 +536,20 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
             ny = sd * (double)(r1 * s1);
             nz = sd * (double)(r2 * c2);
         }
+        // acceptance uniform: the bits of electron 0's draw that its proposal
+        // leaves unused, the low bytes of the four words (Box-Muller reads the
+        // high 24 bits) or, for uniform proposals, the word w; one Philox call
+        // per lane and step
+        const uint32_t spare = a.sw.uniform_move
+                                   ? rnd.w
+                                   : ((rnd.x & 0xFFu) | ((rnd.y & 0xFFu) << 8) | ((rnd.z & 0xFFu) << 16) | (rnd.w << 24));
+        const uint32_t ubits = (uint32_t)__shfl((int)spare, L.wbase);
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
         double lpn = eval_logpsi<T, NM, RAD, AO, WLDS>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
         if (!(lpn > lp_floor)) lpn = lp_floor;
-        // acceptance uniform: counter slot 0x80000000 of the same walker and step;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
-        const u4 ua = philox4x32_10(u4{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, 0x80000000u}, k0, k1);
-        const double U = u01_closed0(ua.x);
+        const double U = u01_closed0(ubits);
         const bool acc = 2.0 * (lpn - lp) >= (U > 0.0 ? (double)logf((float)U) : -INFINITY);
         if (acc) {
             x = xn;

@@ -109,3 +109,23 @@ def test_batch_entry_points_reject_bad_arguments_without_gpu_calls():
     assert lib.qmc_metropolis_sweep_batch(None, N.QMC_PREC_MIXED, None, 1, None) != 0
     assert lib.qmc_wf_eval_batch(None, N.QMC_PREC_MIXED, N.QMC_EVAL_ELOC, None, 1, None) != 0
     assert "null" in lib.qmc_last_error().decode()
+
+
+def test_acceptance_bits_uniform_and_independent():
+    """The sweep's acceptance uniform is assembled from the low bytes of the four
+    Philox words of electron 0 (Box-Muller consumes the high 24 bits of each):
+    on 20000 counters it is uniform (Kolmogorov-Smirnov) and uncorrelated with
+    the four proposal uniforms."""
+    import numpy as np
+
+    n = 20000
+    key = (0x9E3779B9, 12345)
+    words = np.array([philox_host((w, 0, 7, 0), key) for w in range(n)], dtype=np.uint64)
+    u = ((words[:, 0] & 0xFF) | ((words[:, 1] & 0xFF) << 8) | ((words[:, 2] & 0xFF) << 16)
+         | ((words[:, 3] & 0xFF) << 24)).astype(np.float64) * 2.0 ** -32
+    s = np.sort(u)
+    ks = np.max(np.maximum(np.arange(1, n + 1) / n - s, s - np.arange(n) / n))
+    assert ks < 1.63 / np.sqrt(n)  # 1 % level
+    for k in range(4):
+        hi = (words[:, k] >> 8).astype(np.float64) * 2.0 ** -24
+        assert abs(np.corrcoef(u, hi)[0, 1]) < 4.0 / np.sqrt(n)
