@@ -56,18 +56,12 @@ struct Tables {
 };
 
 // ------------------------------------------------------------------ math
-// exp(x) in FP32 as 2^(x log2 e) on the hardware exp2: log2 e = L_hi + L_lo,
-// y = x L_hi rounded, e = (x L_hi - y) + x L_lo (fma gives the rounding error
-// exactly), exp(x) = 2^y 2^e = 2^y (1 + e ln 2 + O(e^2)) with |e| < 2^-16 for
-// |x| < 128; error about 2 ulp.
-__device__ __forceinline__ float fexp(float x) {
-    const float L_hi = 1.44269502162933349609375f;  // log2(e) rounded to float
-    const float L_lo = 1.925963033500011e-8f;       // log2(e) - L_hi
-    const float y = x * L_hi;
-    const float e = fmaf(x, L_hi, -y) + x * L_lo;
-    const float p = __builtin_amdgcn_exp2f(y);
-    return fmaf(p, e * 0.693147182464599609375f, p);
-}
+// exp(x) in FP32 as the hardware 2^y of y = x log2 e: relative error about
+// (|x| + 1) 2^-23, of the order of the rounding already in the argument
+// (x = -a r^2 in FP32 carries |x| 2^-24); a compensated y (x log2 e split into
+// its rounded product and the fma residual) halves the first term for three
+// more instructions per exponential, which the sweeps do not need.
+__device__ __forceinline__ float fexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269502162933349609375f); }
 __device__ __forceinline__ double fexp(double x) { return exp(x); }
 
 // 1/x: the hardware reciprocal (1 ulp) in FP32, a division in FP64
@@ -401,7 +395,9 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
 #pragma unroll
             for (int j = k + 1; j < NM; ++j) pa[j] = __shfl(a[j], plane);
 #pragma unroll
-            for (int j = k + 1; j < NM; ++j) a[j] = is_piv ? pa[j] * ip : a[j] - f * pa[j];
+            // (rows already used are never read again by the determinant-only
+            // elimination, so only the eliminated rows are written there)
+            for (int j = k + 1; j < NM; ++j) a[j] = (WITH_B && is_piv) ? pa[j] * ip : a[j] - f * pa[j];
             if (WITH_B) {
                 double pb[NM];
 #pragma unroll
