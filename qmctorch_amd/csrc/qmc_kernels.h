@@ -522,16 +522,17 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
             nz = a.sw.step_size * (2.0 * u01_closed0(rnd.z) - 1.0);
         } else {
             // Box-Muller on (rnd.x, rnd.y) and (rnd.z, rnd.w), single precision:
-            // 24-bit uniforms in (0,1], radius sqrt(-2 ln u), angle 2 pi v
+            // 24-bit uniforms in (0,1], radius sqrt(-2 ln u) = sqrt(-2 ln2 log2 u),
+            // angle 2 pi v; the hardware log2, sqrt, sin and cos (the last two take
+            // the angle in revolutions), absolute errors below 1e-7
             const float u1 = ((float)(rnd.x >> 8) + 1.0f) * 5.9604644775390625e-8f;
             const float u2 = ((float)(rnd.z >> 8) + 1.0f) * 5.9604644775390625e-8f;
             const float v1 = (float)(rnd.y >> 8) * 5.9604644775390625e-8f;
             const float v2 = (float)(rnd.w >> 8) * 5.9604644775390625e-8f;
-            const float r1 = sqrtf(-2.0f * logf(u1));
-            const float r2 = sqrtf(-2.0f * logf(u2));
-            float s1, c1, s2, c2;
-            sincospif(2.0f * v1, &s1, &c1);
-            sincospif(2.0f * v2, &s2, &c2);
+            const float r1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));
+            const float r2 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
+            const float s1 = __builtin_amdgcn_sinf(v1), c1 = __builtin_amdgcn_cosf(v1);
+            const float c2 = __builtin_amdgcn_cosf(v2);
             nx = sd * (double)(r1 * c1);
             ny = sd * (double)(r1 * s1);
             nz = sd * (double)(r2 * c2);
@@ -550,7 +551,8 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         if (!(lpn > lp_floor)) lpn = lp_floor;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
         const double U = u01_closed0(ubits);
-        const bool acc = 2.0 * (lpn - lp) >= (U > 0.0 ? (double)logf((float)U) : -INFINITY);
+        const bool acc =
+            2.0 * (lpn - lp) >= (U > 0.0 ? 0.6931471805599453 * (double)__builtin_amdgcn_logf((float)U) : -INFINITY);
         if (acc) {
             x = xn;
             y = yn;

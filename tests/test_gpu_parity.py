@@ -143,6 +143,33 @@ def test_sweep_exact_eigenfunction():
     assert 0.05 < s.last_acceptance < 0.999
 
 
+def test_sweep_proposal_distribution():
+    """A nearly flat density (s Gaussian with alpha = 1e-9) accepts every move, so one
+    step displaces each coordinate by the proposal itself: mean 0 and variance
+    sigma2 = step / (2 sqrt(2 ln 2)) per coordinate (metropolis.py:206-212), the
+    same in x, y and z, Gaussian kurtosis 3 -- for normal and uniform-cube moves."""
+    _say("case sweep proposal")
+    from qmctorch_amd.sampler import Metropolis
+    from qmctorch_amd.wavefunction import SlaterJastrow
+
+    mol = systems.hydrogen_atom("gto_pure", 1e-9)
+    wf = SlaterJastrow(mol, jastrow=None, cuda=True)
+    nw = 1 << 17
+    for proba, var, kurt in (("normal", 0.2 / (2 * math.sqrt(2 * math.log(2.0))), 3.0), ("uniform", 0.2 ** 2 / 3, 1.8)):
+        s = Metropolis(nwalkers=nw, nstep=1, step_size=0.2, nelec=1, init=mol.domain("normal"),
+                       move={"type": "all-elec", "proba": proba}, cuda=True, seed=21)
+        x0 = torch.zeros(nw, 3, dtype=torch.float64, device=DEV)
+        d = s(wf.pdf, pos=x0.clone(), with_tqdm=False).detach()
+        assert s.last_acceptance > 0.9999
+        m = d.mean(0)
+        v = d.var(0)
+        k = (d ** 4).mean(0) / v ** 2
+        _say("  %s: mean %s var %s (exact %.6f) kurtosis %s" % (proba, m.tolist(), v.tolist(), var, k.tolist()))
+        assert torch.all(m.abs() < 5 * math.sqrt(var / nw))
+        assert torch.all((v - var).abs() < 5 * var * math.sqrt(2.0 / nw))
+        assert torch.all((k - kurt).abs() < 0.05)
+
+
 def test_sweep_gaussian_moments():
     """|psi|^2 of a normalised s Gaussian (alpha = 0.5) gives <x^2> = 1/(4 alpha)."""
     _say("case sweep moments")
