@@ -485,4 +485,76 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// ---------------------------------------------- determinant of larger matrices
+// log|det| and sign of the n x n matrix whose row r lives in lane gbase + r,
+// for the larger register buckets (NM > 8), with the pivot choice and the row
+// arithmetic of lane_gauss_jordan<NM, false> (so the results are identical)
+// but fewer cross-lane moves per column:
+//  * the pivot (largest |a_rk| among the rows not used yet, first one on ties)
+//    by a segmented max-reduction over the group's lanes: lane r merges the
+//    best of rows [r + off, r + 2 off) for off = 1, 2, 4, ..., keeping its own
+//    on ties, so lane gbase ends with the group's first maximum after
+//    ceil(log2 n) shuffle rounds instead of n shuffles into every lane;
+//  * the pivot row through rowbuf, this group's NM doubles of LDS: the pivot
+//    lane stores its remaining columns (when may_store), the group reads them
+//    back as broadcast loads (two columns per load) instead of one shuffle
+//    pair per column.
+// Rows already used are never read again (f = 0 for them), so only the rows
+// still to be eliminated are written.
+template <int NM>
+__device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, int myrow, int gbase, uint64_t gmask,
+                                             bool may_store, double* rowbuf, double& logdet, double& sgn) {
+    bool done = false;
+    sgn = 1.0;
+    int inversions = 0;
+    bool singular = false;
+    double mant = 1.0;  // |det| = mant * 2^expo, renormalised after every pivot
+    int expo = 0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        if (k < nmax) {
+            const bool colk = k < n;
+            double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+            int brow = myrow;
+#pragma unroll
+            for (int off = 1; off < NM; off <<= 1) {
+                const double ob = __shfl_down(best, off);
+                const int orow = __shfl_down(brow, off);
+                if (myrow + off < n && ob > best) {
+                    best = ob;
+                    brow = orow;
+                }
+            }
+            const int br = __shfl(brow, gbase);
+            const int plane = gbase + br;
+            const uint64_t bal = __ballot(!done && myrow < br);
+            if (colk) inversions += __popcll(bal & gmask);
+            const double piv = __shfl(a[k], plane);
+            const bool is_piv = colk && (myrow == br);
+            if (colk) {
+                singular = singular || (piv == 0.0);
+                int ex;
+                mant = frexp(mant * fabs(piv), &ex);
+                expo += ex;
+                sgn *= (piv < 0.0) ? -1.0 : 1.0;
+            }
+            const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+            const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
+            if (k + 1 < NM) {
+                wave_lds_sync();  // the previous column's reads are done
+                if (is_piv && may_store) {
+#pragma unroll
+                    for (int j = k + 1; j < NM; ++j) rowbuf[j] = a[j];
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
+            }
+            if (is_piv) done = true;
+        }
+    }
+    if (inversions & 1) sgn = -sgn;
+    logdet = singular ? -INFINITY : log_mantissa(mant) + 0.6931471805599453 * (double)expo;
+}
+
 }  // namespace qmc

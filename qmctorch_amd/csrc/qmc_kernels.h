@@ -17,6 +17,7 @@
 namespace qmc {
 
 constexpr int kBlock = 256;
+constexpr int kBlockWaves = kBlock / 64;
 
 struct WalkerLane {
     int lane, wave, G, wl, wbase, e, w, wclamp, spin, myrow, gbase, n_sig;
@@ -194,8 +195,11 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
     }
 }
 
-// Value of ln J = 1/2 sum_{i != j} B_ee + sum_A B_en seen from one electron
-// (summed over the walker's lanes by the caller), in the bulk precision T.
+// This electron's share of ln J = sum_{i<j} B_ee + sum_A B_en (summed over the
+// walker's lanes by the caller), in the bulk precision T.  Each e-e pair is
+// evaluated once: lane e takes the partners e+1 .. e+H (mod Ne), H = Ne/2, and
+// for even Ne the pair at distance H, which both of its lanes see, at half
+// weight.
 template <typename T>
 __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
                                            double y, double z) {
@@ -203,24 +207,30 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
     if (t.use_ee) {
         const T wee = (T)ldc(prm.w_ee, 0);
         const T xs = (T)x, ys = (T)y, zs = (T)z;
-        // partner coordinates fetched four electrons at a time (independent shuffles)
-        for (int j0 = 0; j0 < t.nelec; j0 += 4) {
+        const int Ne = t.nelec, H = t.nelec / 2;
+        const bool even = (Ne & 1) == 0;
+        // partner coordinates fetched four at a time (independent shuffles)
+        for (int d0 = 1; d0 <= H; d0 += 4) {
             T xj[4], yj[4], zj[4];
+            int jj[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int jl = L.wbase + (j0 + u < t.nelec ? j0 + u : 0);
-                xj[u] = __shfl(xs, jl);
-                yj[u] = __shfl(ys, jl);
-                zj[u] = __shfl(zs, jl);
+                const int d = d0 + u <= H ? d0 + u : 1;
+                const int j = L.e + d < Ne ? L.e + d : L.e + d - Ne;
+                jj[u] = j;
+                xj[u] = __shfl(xs, L.wbase + j);
+                yj[u] = __shfl(ys, L.wbase + j);
+                zj[u] = __shfl(zs, L.wbase + j);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int j = j0 + u;
-                if (j < t.nelec) {
+                const int d = d0 + u;
+                if (d <= H) {
                     const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
                     const T r = fsqrt(dx * dx + dy * dy + dz * dz);
-                    const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
-                    k += (j != L.e) ? w0 * r * frcp(T(1) + wee * r) : T(0);
+                    const T w0 = ((jj[u] < t.nup) == (L.e < t.nup)) ? T(0.25) : T(0.5);
+                    const T wt = (even && d == H) ? T(0.5) * w0 : w0;
+                    k += wt * r * frcp(T(1) + wee * r);
                 }
             }
         }
@@ -237,6 +247,16 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
     return k;
 }
 
+// Pivot-row buffer of lane_det_lds for the group of this lane: one NM-double
+// row per lane of the wavefront, used by the group whose first row is that
+// lane (gbase, 0..63), so any lane layout fits.  Lanes past the wavefront's
+// last walker compute a throw-away group and do not store.
+template <int NM>
+__device__ __forceinline__ double* det_rowbuf(const WalkerLane& L) {
+    __shared__ double rows[kBlockWaves * 64 * NM];
+    return rows + ((threadIdx.x >> 6) * 64 + L.gbase) * NM;
+}
+
 // Single configuration, single CI term: log|D_up D_dn| and its sign (ground state).
 template <int NM, typename T>
 __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& prm, const WalkerLane& L,
@@ -244,9 +264,14 @@ __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& 
     double a[NM], b[NM];
 #pragma unroll
     for (int j = 0; j < NM; ++j) a[j] = (double)mo[j];
-    double lg, sg, tp;
-    int col;
-    lane_gauss_jordan<NM, false>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    double lg, sg;
+    if constexpr (NM > 8) {
+        lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg);
+    } else {
+        double tp;
+        int col;
+        lane_gauss_jordan<NM, false>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    }
     const int dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
     const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
     const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
@@ -301,9 +326,14 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
         double a[NM], b[NM];
         gather_conf<NM>(t, L, c, mo, a);
         if (KIN) gather_conf<NM>(t, L, c, bk, b);
-        double lg, sg, tp;
-        int col;
-        lane_gauss_jordan<NM, KIN>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        double lg, sg, tp = 0.0;
+        if constexpr (!KIN && NM > 8) {
+            lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg,
+                             sg);
+        } else {
+            int col;
+            lane_gauss_jordan<NM, KIN>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        }
         const double tr = KIN ? seg_sum(tp, L.gbase, L.n_sig, t.nsig_max) : 0.0;
 #pragma unroll
         for (int c2 = 0; c2 < QMC_MAXCONF; ++c2) {
@@ -650,8 +680,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
 }
 
 // ------------------------------------------------------------------ drift
-// ln J seen from one electron (jastrow_value's arithmetic) and the gradient
-// of ln J with respect to this electron, in the bulk precision T:
+// This electron's half of its e-e pairs plus its e-n terms, summing to ln J
+// over the walker's lanes (every pair seen from both ends at half weight), and
+// the gradient of ln J with respect to this electron, in the bulk precision T:
 // d/dr [w0 r / (1 + w r)] = w0 / (1 + w r)^2 (pade_jastrow_kernel.py of both
 // Jastrow families), the e-e pair weight w0 entering the gradient in full.
 template <typename T>

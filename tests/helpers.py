@@ -50,6 +50,10 @@ CASES = {
     "lih_631g_cas22_ee_en": lambda: (systems.lih(), dict(configs="cas(2,2)", w_ee=0.8, w_en=1.2,
                                                           ci=[1.0, 0.1, -0.2, 0.05])),
     "h2o_ccpvdz_ee_en": lambda: (systems.h2o(), dict(w_ee=1.0, w_en=1.0)),
+    # 4 electrons with MO columns up to 9: register bucket 12 at 16 walkers per wavefront (the
+    # larger-bucket elimination with group leaders on all lanes), 64 CI terms, 8 unique per spin
+    "lih_631g_cas28_ee": lambda: (systems.lih(), dict(configs="cas(2,8)", w_ee=0.8,
+                                                       ci=[1.0] + list(0.2 * np.random.default_rng(7).standard_normal(63)))),
     "n2_ccpvdz_ee": lambda: (systems.n2(), dict(w_ee=0.9)),
     "lih_adf_sto_sd22": lambda: (systems.lih_adf(), dict(configs="single_double(2,2)", w_ee=0.6,
                                                           ci=[0.9, 0.2, 0.1, 0.4])),

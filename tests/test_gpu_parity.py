@@ -188,6 +188,37 @@ def test_sweep_gaussian_moments():
     assert abs(m - 1 / (4 * alpha)) < 6 * err
 
 
+@pytest.mark.parametrize("case", ["lih_631g_cas28_ee", "c6h6_ccpvdz_ee_en"])
+def test_sweep_logpsi_matches_fp64_evaluation(case):
+    """The log|psi| a mixed-precision sweep keeps for its walkers equals an FP64
+    evaluation at the final positions (register buckets above 8: the LDS-row
+    elimination, with many walkers per wavefront for LiH CAS(2,8))."""
+    _say("case sweep logpsi", case)
+    from qmctorch_amd import _native as N
+
+    mol, kw = helpers.CASES[case]()
+    wf, _ = helpers.make_pair(mol, precision="mixed", **kw)
+    nw = 96 if mol.nelec > 20 else 700
+    x = helpers.positions(mol, nw, seed=31).to(DEV).contiguous()
+    la = torch.empty(nw, dtype=torch.float64, device=DEV)
+    sg = torch.empty_like(la)
+    na = torch.zeros(nw, dtype=torch.int32, device=DEV)
+    wk = N.QmcWalkers()
+    wk.pos, wk.logabs, wk.sign, wk.naccept, wk.walker_id = x.data_ptr(), la.data_ptr(), sg.data_ptr(), na.data_ptr(), 0
+    sw = N.QmcSweepDesc()
+    sw.step0, sw.nstep, sw.init_logpsi = 0, 20, 1
+    sw.sigma2, sw.seed, sw.step_size, sw.keep_every = 0.2 / (2 * math.sqrt(2 * math.log(2.0))), 5, 0.2, 1
+    prm, keep = wf._params("mixed")
+    N.check(N.load().qmc_metropolis_sweep(wf.device_plan().handle, prm, N.QMC_PREC_MIXED, wk, nw, sw,
+                                          N.current_stream()))
+    l64, s64 = wf.log_psi(x, "fp64")
+    d = (la - l64).abs()
+    _say("  accepted %d, max |log|psi| - FP64| %.2e" % (int(na.sum()), float(d.max())))
+    assert int(na.sum()) > 0
+    assert float(d.max()) < 1e-3 * max(1.0, float(l64.abs().max()) / 50)
+    assert torch.equal(sg, s64)
+
+
 def test_sweep_batch_invariance():
     """A walker's chain depends on (seed, walker id, step) only, not on the batch."""
     _say("case sweep batch invariance")
