@@ -485,16 +485,16 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// ---------------------------------------------- determinant of larger matrices
+// ------------------------------------------------- determinant-only elimination
 // log|det| and sign of the n x n matrix whose row r lives in lane gbase + r,
-// for the larger register buckets (NM > 8), with the pivot choice and the row
-// arithmetic of lane_gauss_jordan<NM, false> (so the results are identical)
-// but fewer cross-lane moves per column:
-//  * the pivot (largest |a_rk| among the rows not used yet, first one on ties)
-//    by a segmented max-reduction over the group's lanes: lane r merges the
-//    best of rows [r + off, r + 2 off) for off = 1, 2, 4, ..., keeping its own
-//    on ties, so lane gbase ends with the group's first maximum after
-//    ceil(log2 n) shuffle rounds instead of n shuffles into every lane;
+// with the pivot choice and the row arithmetic of lane_gauss_jordan<NM, false>
+// (so the results are identical) but fewer cross-lane moves per column:
+//  * the pivot (largest |a_rk| among the rows not used yet, first one on ties):
+//    up to 8 columns from the n candidates gathered into every lane; for the
+//    larger buckets by a segmented max-reduction over the group's lanes: lane
+//    r merges the best of rows [r + off, r + 2 off) for off = 1, 2, 4, ...,
+//    keeping its own on ties, so lane gbase ends with the group's first
+//    maximum after ceil(log2 n) shuffle rounds instead of n shuffles;
 //  * the pivot row through rowbuf, this group's NM doubles of LDS: the pivot
 //    lane stores its remaining columns (when may_store), the group reads them
 //    back as broadcast loads (two columns per load) instead of one shuffle
@@ -514,18 +514,35 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
     for (int k = 0; k < NM; ++k) {
         if (k < nmax) {
             const bool colk = k < n;
-            double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
-            int brow = myrow;
+            int br;
+            if constexpr (NM <= 8) {
+                const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+                double c[NM];
 #pragma unroll
-            for (int off = 1; off < NM; off <<= 1) {
-                const double ob = __shfl_down(best, off);
-                const int orow = __shfl_down(brow, off);
-                if (myrow + off < n && ob > best) {
-                    best = ob;
-                    brow = orow;
+                for (int s = 0; s < NM; ++s) c[s] = __shfl(cand, gbase + (s < n ? s : 0));
+                double bb = -1.0;
+                br = 0;
+#pragma unroll
+                for (int s = 0; s < NM; ++s) {
+                    if (s < n && c[s] > bb) {
+                        bb = c[s];
+                        br = s;
+                    }
                 }
+            } else {
+                double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+                int brow = myrow;
+#pragma unroll
+                for (int off = 1; off < NM; off <<= 1) {
+                    const double ob = __shfl_down(best, off);
+                    const int orow = __shfl_down(brow, off);
+                    if (myrow + off < n && ob > best) {
+                        best = ob;
+                        brow = orow;
+                    }
+                }
+                br = __shfl(brow, gbase);
             }
-            const int br = __shfl(brow, gbase);
             const int plane = gbase + br;
             const uint64_t bal = __ballot(!done && myrow < br);
             if (colk) inversions += __popcll(bal & gmask);

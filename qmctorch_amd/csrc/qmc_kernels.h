@@ -261,17 +261,11 @@ __device__ __forceinline__ double* det_rowbuf(const WalkerLane& L) {
 template <int NM, typename T>
 __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& prm, const WalkerLane& L,
                                               const T (&mo)[NM], double logc0, double& logabs, double& sgn) {
-    double a[NM], b[NM];
+    double a[NM];
 #pragma unroll
     for (int j = 0; j < NM; ++j) a[j] = (double)mo[j];
     double lg, sg;
-    if constexpr (NM > 8) {
-        lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg);
-    } else {
-        double tp;
-        int col;
-        lane_gauss_jordan<NM, false>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
-    }
+    lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg);
     const int dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
     const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
     const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
@@ -327,7 +321,7 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
         gather_conf<NM>(t, L, c, mo, a);
         if (KIN) gather_conf<NM>(t, L, c, bk, b);
         double lg, sg, tp = 0.0;
-        if constexpr (!KIN && NM > 8) {
+        if constexpr (!KIN) {
             lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg,
                              sg);
         } else {
