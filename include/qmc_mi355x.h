@@ -32,6 +32,11 @@ enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 }; /* MIXED: FP32 AO/MO/Jastrow, FP
  * configurations per spin handled by the kernels. */
 #define QMC_MAXP 12
 #define QMC_MAXCONF 8
+/* Plans with more unique configurations per spin (up to QMC_MAXEXC) must be
+ * single and double excitations of the ground state (columns 0..n-1); psi, pdf,
+ * the local energy and the Metropolis sweep then use the excitation
+ * (trace-trick) kernels, the other entry points return QMC_ERR_UNSUPPORTED. */
+#define QMC_MAXEXC 1024
 
 /*
  * Static description of one Slater-Jastrow wavefunction (host pointers; the

@@ -20,6 +20,7 @@ LIB = os.path.join(PKG, "libqmcmi355x.so")
 # (source, object tag, extra defines): kernel families are compiled once per radial type
 _FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip", "qmc_grad.hip",
              "qmc_batch_sweep_f32.hip", "qmc_batch_sweep_f64.hip", "qmc_batch_wf_f32.hip", "qmc_batch_wf_f64.hip",
+             "qmc_exc.hip",
 ]
 # HMC / drift kernels: single-configuration (MC=1) and multi-configuration (MC=8) instances
 _HMC = [("qmc_hmc_f32.hip", mc) for mc in (1, 8)] + [("qmc_hmc_f64.hip", mc) for mc in (1, 8)]

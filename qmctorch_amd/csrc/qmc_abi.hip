@@ -110,6 +110,68 @@ uint64_t layout_signature(const QmcSystemDesc* d) {
 }  // namespace
 
 namespace {
+// Excitation table of nconf configurations of n columns each (row c of conf
+// lists its MO columns in the order of the determinant): per configuration
+// {level, h1, h2, p1, p2, sign} with the holes h (ground-state columns 0..n-1
+// missing, ascending), the particles p (columns >= n, ascending, stored as
+// p - n) and the sign of the listed order relative to the ground-state order
+// with h_i replaced in place by p_i.  False when a configuration is more than
+// a double excitation of the ground state or repeats a column.
+bool excitation_table(const int32_t* conf, int nconf, int n, std::vector<int32_t>& out) {
+    out.assign((size_t)nconf * 6, 0);
+    for (int c = 0; c < nconf; ++c) {
+        const int32_t* cc = conf + (size_t)c * n;
+        int holes[2] = {0, 0}, parts[2] = {0, 0}, nh = 0, np = 0;
+        for (int g = 0; g < n; ++g) {
+            bool in = false;
+            for (int j = 0; j < n; ++j) in = in || cc[j] == g;
+            if (!in) {
+                if (nh == 2) return false;
+                holes[nh++] = g;
+            }
+        }
+        for (int j = 0; j < n; ++j) {
+            if (cc[j] < 0) return false;
+            if (cc[j] >= n) {
+                if (np == 2) return false;
+                parts[np++] = cc[j];
+            }
+        }
+        if (nh != np) return false;
+        if (np == 2 && parts[0] > parts[1]) std::swap(parts[0], parts[1]);
+        if (np == 2 && parts[0] == parts[1]) return false;
+        std::vector<int> inplace(n);
+        for (int g = 0; g < n; ++g) inplace[g] = g;
+        for (int i = 0; i < nh; ++i) inplace[holes[i]] = parts[i];
+        std::vector<int> perm(n, -1);
+        for (int j = 0; j < n; ++j)
+            for (int g = 0; g < n; ++g)
+                if (inplace[g] == cc[j]) perm[j] = g;
+        int inv = 0;
+        for (int i = 0; i < n; ++i) {
+            if (perm[i] < 0) return false;
+            for (int j = i + 1; j < n; ++j) inv += perm[i] > perm[j];
+        }
+        int32_t* o = out.data() + (size_t)c * 6;
+        o[0] = nh;
+        o[1] = holes[0];
+        o[2] = holes[1];
+        o[3] = nh > 0 ? parts[0] - n : 0;
+        o[4] = nh > 1 ? parts[1] - n : 0;
+        o[5] = (inv & 1) ? -1 : 1;
+    }
+    return true;
+}
+
+// Entry points without the trace-trick path refuse plans that need it.
+int refuse_excitations(const QmcPlan* plan) {
+    if (plan && plan->t.exc_mode)
+        return fail(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin: this entry point has no "
+                                         "excitation (trace-trick) path (psi, pdf, local energy and the Metropolis "
+                                         "sweep have)");
+    return QMC_OK;
+}
+
 int launch_hmc(const HArgs& a, int precision, hipStream_t s) {
     const bool f64 = precision == QMC_PREC_FP64, sto = a.t.radial == QMC_RADIAL_STO;
     if (a.t.nconf_max <= 1) {
@@ -136,8 +198,22 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
     if (d->nup + d->ndown != d->nelec || d->ndown > d->nup) return fail(QMC_ERR_ARG, "inconsistent spin counts");
     if (d->radial_type != QMC_RADIAL_GTO_PURE && d->radial_type != QMC_RADIAL_STO)
         return fail(QMC_ERR_UNSUPPORTED, "radial type must be gto_pure or sto");
-    if (d->nconf_up < 1 || d->nconf_up > QMC_MAXCONF || d->nconf_dn < 1 || d->nconf_dn > QMC_MAXCONF)
-        return fail(QMC_ERR_UNSUPPORTED, "unique configurations per spin must be in [1, 8]");
+    if (d->nconf_up < 1 || d->nconf_up > QMC_MAXEXC || d->nconf_dn < 1 || d->nconf_dn > QMC_MAXEXC)
+        return fail(QMC_ERR_UNSUPPORTED, "unique configurations per spin must be in [1, 1024]");
+    // beyond QMC_MAXCONF per spin: single and double excitations of the ground
+    // state through the trace-trick kernels
+    const bool exc = d->nconf_up > QMC_MAXCONF || d->nconf_dn > QMC_MAXCONF;
+    std::vector<int32_t> exc_up, exc_dn;
+    if (exc) {
+        if (!excitation_table(d->conf_up, d->nconf_up, d->nup, exc_up) ||
+            !excitation_table(d->conf_dn, d->ndown > 0 ? d->nconf_dn : 0, d->ndown, exc_dn))
+            return fail(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin are supported as single and "
+                                             "double excitations of the ground state only");
+        if (exc_dn.empty()) exc_dn.assign(6, 0);
+        if (d->nelec < 2) return fail(QMC_ERR_UNSUPPORTED, "the excitation path needs at least two electrons");
+        if (std::max(d->nmo, std::max(d->nup, d->ndown)) > 16)
+            return fail(QMC_ERR_UNSUPPORTED, "the excitation path supports up to 16 MO columns per electron");
+    }
     if (d->nmo < d->nup) return fail(QMC_ERR_ARG, "nmo smaller than nup");
     for (int g = 0; g < d->ngroup; ++g)
         if (d->grp_nprim[g] > QMC_MAXP || d->grp_nprim[g] < 1) return fail(QMC_ERR_UNSUPPORTED, "group too large");
@@ -195,7 +271,13 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.ci_up = (decltype(t.ci_up))p->upload(d->ci_up, d->nci);
         t.ci_dn = (decltype(t.ci_dn))p->upload(d->ci_dn, d->nci);
         if (!t.conf_dn) t.conf_dn = t.conf_up;
-        p->spec = find_spec(layout_signature(d), nm_bucket_fine(t));
+        t.exc_mode = exc ? 1 : 0;
+        if (exc) {
+            t.exc_up = (decltype(t.exc_up))p->upload(exc_up.data(), exc_up.size());
+            t.exc_dn = (decltype(t.exc_dn))p->upload(exc_dn.data(), exc_dn.size());
+        }
+        // generated kernels have no excitation path
+        p->spec = exc ? nullptr : find_spec(layout_signature(d), nm_bucket_fine(t));
     } catch (const std::exception& ex) {
         delete p;
         return fail(QMC_ERR_HIP, ex.what());
@@ -253,8 +335,12 @@ int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision,
     a.nw = nw;
     a.out = *out;
     hipStream_t s = (hipStream_t)stream;
-    const bool sto = plan->t.radial == QMC_RADIAL_STO;
-    (void)sto;
+    if (plan->t.exc_mode) {
+        rc = QMC_BY_RAD(launch_wf_exc, a.t.radial, a, precision, mode, s);
+        if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     if (plan->spec && plan->use_spec && plan->spec->wf(a, precision, mode, s) == 0) {
         QMC_HIP(hipGetLastError());
         return QMC_OK;
@@ -281,8 +367,12 @@ int qmc_metropolis_sweep(const QmcPlan* plan, const QmcParams* params, int32_t p
     a.sw = *sw;
     a.nw = nw;
     hipStream_t s = (hipStream_t)stream;
-    const bool sto = plan->t.radial == QMC_RADIAL_STO;
-    (void)sto;
+    if (plan->t.exc_mode) {
+        rc = QMC_BY_RAD(launch_sweep_exc, a.t.radial, a, precision, s);
+        if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
     if (plan->spec && plan->use_spec && plan->spec->sweep(a, precision, s) == 0) {
         QMC_HIP(hipGetLastError());
         return QMC_OK;
@@ -298,6 +388,7 @@ int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precisio
                   const QmcHmcDesc* hm, qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
     if (!wk || !hm || !wk->pos || !wk->logabs || !wk->sign) return fail(QMC_ERR_ARG, "null walker state");
     if (hm->traj_length < 1) return fail(QMC_ERR_ARG, "traj_length must be >= 1");
     if (hm->keep && hm->keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
@@ -326,6 +417,7 @@ int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision
                  double* drift, double* logabs, double* sign, qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
     if (!pos || !drift) return fail(QMC_ERR_ARG, "null pos/drift");
     if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
     if (nw == 0) return QMC_OK;
@@ -379,6 +471,7 @@ int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const dou
                          qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
     if (!pos || !ao_rows || !gamma_rows || !dci || !djas) return fail(QMC_ERR_ARG, "null output");
     if (nw == 0) return QMC_OK;
     GArgs a;
@@ -412,6 +505,7 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
                        double* djas, double* drift, qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
     if (!pos) return fail(QMC_ERR_ARG, "null pos");
     if ((ao_rows == nullptr) != (b_rows == nullptr)) return fail(QMC_ERR_ARG, "ao_rows and b_rows go together");
     if (nw == 0) return QMC_OK;
@@ -550,6 +644,7 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
         const QmcSweepJob& x = jobs[j];
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
+        if ((rc = refuse_excitations(x.plan))) return rc;
         if (!x.walkers.pos || !x.walkers.logabs || !x.walkers.sign) return fail(QMC_ERR_ARG, "null walker state");
         if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.sweep.keep && x.sweep.keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
@@ -612,6 +707,7 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
         const QmcEvalJob& x = jobs[j];
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
+        if ((rc = refuse_excitations(x.plan))) return rc;
         if (!x.pos) return fail(QMC_ERR_ARG, "null pos");
         if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.nw > 0) live.push_back(x);

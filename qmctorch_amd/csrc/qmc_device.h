@@ -33,6 +33,7 @@ struct Tables {
     int nelec, nup, ndown, natom, nao, nprim_ref, radial;
     int ngroup, nentry, nmo, nci, nconf_up, nconf_dn, nconf_max, nsig_max;
     int use_ee, use_en, identity_conf;
+    int exc_mode;  // configurations as single/double excitations (trace-trick kernels)
     double vnn;
     const QMC_CONST int* grp_atom;
     const QMC_CONST int* grp_nprim;
@@ -53,6 +54,8 @@ struct Tables {
     const QMC_CONST int* conf_dn;
     const QMC_CONST int* ci_up;
     const QMC_CONST int* ci_dn;
+    const QMC_CONST int* exc_up;  // [nconf_up][6] {level, h1, h2, p1, p2, sign}, or null
+    const QMC_CONST int* exc_dn;
 };
 
 // ------------------------------------------------------------------ math

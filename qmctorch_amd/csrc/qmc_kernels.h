@@ -19,6 +19,10 @@ namespace qmc {
 constexpr int kBlock = 256;
 constexpr int kBlockWaves = kBlock / 64;
 
+// Dynamic LDS of the kernels that stage rows per block (W rows of generated
+// sweeps, the excitation kernels' M / X / Mk rows).
+extern __shared__ __attribute__((aligned(16))) unsigned char qmc_dyn_lds[];
+
 struct WalkerLane {
     int lane, wave, G, wl, wbase, e, w, wclamp, spin, myrow, gbase, n_sig;
     bool lane_real, valid;
@@ -399,6 +403,127 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
     kin = KIN ? st / s : 0.0;
 }
 
+// ------------------------------------------------- excitation (trace trick)
+// Configurations given as single and double excitations of the ground state
+// (slater_pooling.py:147-260 det_unique_single_double, 411-825
+// operator_unique_single_double): with A the ground-state block (columns
+// 0..n-1), V the virtual columns, B the kinetic rows and
+//   M = A^-1 V,  X = A^-1 B,  Mk = X_virt - X_occ M,  t0 = tr(A^-1 B_occ),
+// the excitation replacing columns h_i by p_i (in place) has
+//   single: D/D0 = M[h][p],           tr(A'^-1 B') = t0 + Mk[h][p] / M[h][p]
+//   double: D/D0 = det T, T = M[h][p] (2x2),  tr(A'^-1 B') = t0 + tr(T^-1 Mk[h][p])
+// times the sign of the configuration's listed column order.  The lanes that
+// pivoted hold the rows of M and X after the eliminations of [A | A V] and
+// [A | B]; they store them, and the rows of Mk, into their group's LDS block,
+// from which every lane of the walker evaluates a share of the CI terms.
+
+// R = sign * D/D0 and Q = R * tr(A'^-1 B') of excitation e of one spin group
+// (Mb: M rows stored as full MO rows, so M[h][p] = Mb[h][n + p]; Kb: Mk rows).
+template <int NM, bool KIN>
+__device__ __forceinline__ void exc_ratio(const QMC_CONST int* e, const double* Mb, const double* Kb, int n,
+                                          double t0, double& R, double& Q) {
+    const int lvl = e[0], h1 = e[1], h2 = e[2], p1 = e[3], p2 = e[4];
+    const double sg = (double)e[5];
+    if (lvl == 0) {
+        R = sg;
+        Q = KIN ? sg * t0 : 0.0;
+    } else if (lvl == 1) {
+        const double m = Mb[h1 * NM + n + p1];
+        R = sg * m;
+        Q = KIN ? sg * (m * t0 + Kb[h1 * NM + p1]) : 0.0;
+    } else {
+        const double t11 = Mb[h1 * NM + n + p1], t12 = Mb[h1 * NM + n + p2];
+        const double t21 = Mb[h2 * NM + n + p1], t22 = Mb[h2 * NM + n + p2];
+        const double det = t11 * t22 - t12 * t21;
+        R = sg * det;
+        if (KIN) {
+            const double k11 = Kb[h1 * NM + p1], k12 = Kb[h1 * NM + p2];
+            const double k21 = Kb[h2 * NM + p1], k22 = Kb[h2 * NM + p2];
+            // det(T) tr(T^-1 K) = t22 k11 - t12 k21 - t21 k12 + t11 k22
+            Q = sg * (det * t0 + (t22 * k11 - t12 * k21 - t21 * k12 + t11 * k22));
+        } else {
+            Q = 0.0;
+        }
+    }
+}
+
+// log|sum_n c_n D_n|, its sign and (KIN) the Jacobi kinetic ratio for plans
+// with exc_mode.  lds: the wavefront's block of 2 (G + 1) groups of 3 x
+// nsig_max x NM doubles (M, X and Mk rows), group (walker wl, spin).
+template <int NM, bool KIN, typename T>
+__device__ __forceinline__ void slater_exc(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                           const T (&mo)[NM], const double (&bk)[NM], double* lds, double& logabs,
+                                           double& sgn, double& kin) {
+    const int n = L.n_sig;
+    const int blk = t.nsig_max * NM;  // doubles per row block
+    double* Mb = lds + (size_t)(L.wl * 2 + L.spin) * 3 * blk;
+    double* Xb = Mb + blk;
+    double* Kb = Xb + blk;
+    double a[NM], b[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? (double)mo[j] : 0.0;
+        b[j] = (double)mo[j];
+    }
+    double lg, sg, tp;
+    int col;
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    wave_lds_sync();  // the previous evaluation's reads are done
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Mb[col * NM + j] = b[j];  // row col of A^-1 [A | V]
+    }
+    double t0 = 0.0;
+    if (KIN) {
+        // the same elimination of [A | B]: identical pivots, X = A^-1 B
+        double lg2, sg2;
+        int col2;
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            a[j] = j < n ? (double)mo[j] : 0.0;
+            b[j] = bk[j];
+        }
+        lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg2, sg2, tp, col2);
+        t0 = seg_sum(tp, L.gbase, n, t.nsig_max);  // tr(A^-1 B_occ)
+        if (L.lane_real && col2 >= 0) {
+#pragma unroll
+            for (int j = 0; j < NM; ++j) Xb[col2 * NM + j] = b[j];
+        }
+        wave_lds_sync();
+        // row col of Mk: X[col][n + p] - sum_j X[col][j] M[j][n + p]
+        if (L.lane_real && col >= 0) {
+            for (int p = 0; n + p < t.nmo; ++p) {
+                double v = Xb[col * NM + n + p];
+                for (int j = 0; j < n; ++j) v -= Xb[col * NM + j] * Mb[j * NM + n + p];
+                Kb[col * NM + p] = v;
+            }
+        }
+    }
+    wave_lds_sync();
+    // walker-level combination: lane e of the walker takes CI terms e, e + Ne, ...
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    const double lu = __shfl(lg, up_lane), su = __shfl(sg, up_lane);
+    const double ld = __shfl(lg, dn_lane), sd = __shfl(sg, dn_lane);
+    const double tu = __shfl(t0, up_lane), td = __shfl(t0, dn_lane);
+    const double* Mu = lds + (size_t)(L.wl * 2) * 3 * blk;
+    const double* Md = Mu + 3 * blk;
+    double S = 0.0, ST = 0.0;
+    for (int m = L.e; m < t.nci; m += t.nelec) {
+        const int cu = t.ci_up[m], cd = t.ci_dn[m];
+        double Ru, Qu, Rd = 1.0, Qd = 0.0;
+        exc_ratio<NM, KIN>(t.exc_up + 6 * cu, Mu, Mu + 2 * blk, t.nup, tu, Ru, Qu);
+        if (t.ndown > 0) exc_ratio<NM, KIN>(t.exc_dn + 6 * cd, Md, Md + 2 * blk, t.ndown, td, Rd, Qd);
+        const double c = ldc(prm.ci, m);
+        S += c * Ru * Rd;
+        if (KIN) ST += c * (Qu * Rd + Ru * Qd);
+    }
+    S = seg_sum(S, L.wbase, t.nelec, t.nelec);
+    if (KIN) ST = seg_sum(ST, L.wbase, t.nelec, t.nelec);
+    logabs = lu + (t.ndown > 0 ? ld : 0.0) + log(fabs(S));
+    sgn = su * (t.ndown > 0 ? sd : 1.0) * (S < 0.0 ? -1.0 : 1.0);
+    kin = KIN ? ST / S : 0.0;
+}
+
 struct KArgs {
     Tables t;
     QmcParams p;
@@ -408,7 +533,7 @@ struct KArgs {
 };
 
 // psi / kinetic / local energy of the walkers of one wavefront.
-template <typename T, int NM, int RAD, bool ELOC, typename AO = AoGeneric>
+template <typename T, int NM, int RAD, bool ELOC, typename AO = AoGeneric, bool EXC = false>
 __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) {
     const Tables& t = a.t;
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
@@ -435,7 +560,10 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
                             l2 * (double)mo[m]);
     }
     double logabs, sgn, kin;
-    slater_combine<NM, ELOC>(t, a.p, L, mo, bk, logabs, sgn, kin);
+    if constexpr (EXC)
+        slater_exc<NM, ELOC>(t, a.p, L, mo, bk, (double*)qmc_dyn_lds, logabs, sgn, kin);
+    else
+        slater_combine<NM, ELOC>(t, a.p, L, mo, bk, logabs, sgn, kin);
     const double logj = seg_sum(0.5 * kee + ken, L.wbase, t.nelec, t.nelec);
     double epot = 0.0;
     if (ELOC) epot = seg_sum(vpot, L.wbase, t.nelec, t.nelec) + t.vnn;
@@ -457,6 +585,15 @@ __global__ __launch_bounds__(kBlock) void wf_kernel(KArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;  // whole wavefront beyond the batch
     wf_walkers<T, NM, RAD, ELOC, AO>(a, L);
+}
+
+// Kernels of exc_mode plans: one wavefront per block, dynamic LDS of
+// 2 (G + 1) * 3 * nsig_max * NM doubles (exc_lds_bytes).
+template <typename T, int NM, int RAD, bool ELOC>
+__global__ __launch_bounds__(64) void wf_kernel_exc(KArgs a) {
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    wf_walkers<T, NM, RAD, ELOC, AoGeneric, true>(a, L);
 }
 
 // Many systems in one launch: job j owns global waves [wave0[j], wave0[j+1]).
@@ -484,11 +621,8 @@ struct SArgs {
     int64_t nw;
 };
 
-// Dynamic LDS of the kernels that stage parameter rows per block.
-extern __shared__ __attribute__((aligned(16))) unsigned char qmc_dyn_lds[];
-
 // wlds: with WLDS, this wavefront's LDS copy of the entry-ordered W rows
-template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false>
+template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false, bool EXC = false>
 __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
                                               double y, double z, double logc0, double& sgn,
                                               const T* wlds = nullptr) {
@@ -500,7 +634,10 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
     const T kj = jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
-    if (t.identity_conf) {
+    if constexpr (EXC) {
+        double bk[NM], kin;
+        slater_exc<NM, false>(t, prm, L, mo, bk, (double*)qmc_dyn_lds, logabs, sgn, kin);
+    } else if (t.identity_conf) {
         slater_single<NM>(t, prm, L, mo, logc0, logabs, sgn);
     } else {
         double bk[NM], kin;
@@ -512,7 +649,7 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
 
 // Persistent Metropolis sweep: nstep all-electron moves per launch, walker
 // state in registers (sampler/metropolis.py:134-165, 256-298).
-template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false>
+template <typename T, int NM, int RAD, typename AO = AoGeneric, bool WLDS = false, bool EXC = false>
 __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& L, const T* wlds = nullptr) {
     const Tables& t = a.t;
     const int Ne = t.nelec;
@@ -522,14 +659,17 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     double lp;
     const double logc0 = log(fabs(ldc(a.p.ci, 0)));  // single-configuration CI weight, once per launch
     if (a.sw.init_logpsi) {
-        lp = eval_logpsi<T, NM, RAD, AO, WLDS>(t, a.p, L, x, y, z, logc0, sgn, wlds);
+        lp = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, L, x, y, z, logc0, sgn, wlds);
     } else {
         lp = a.wk.logabs[L.wclamp];
         sgn = a.wk.sign[L.wclamp];
     }
-    // psi == 0 is treated as psi^2 = eps (metropolis.py:103-107,124)
-    const double lp_floor = -18.420680743952367;  // 0.5 * log(1e-16)
-    if (!(lp > lp_floor)) lp = lp_floor;
+    // psi^2 == 0 is treated as psi^2 = eps = 1e-16 (metropolis.py:103-107,124,146,160):
+    // only a density that is zero in FP64 (psi^2 below the smallest subnormal,
+    // or a node) is replaced; a NaN stays NaN and its moves are rejected
+    const double lp_zero = -372.2200359606906;     // 0.5 * log(4.94e-324)
+    const double lp_eps = -18.420680743952367;     // 0.5 * log(1e-16)
+    if (lp < lp_zero) lp = lp_eps;
     const uint64_t wid = a.wk.walker_id ? (uint64_t)a.wk.walker_id[L.wclamp] : (uint64_t)L.wclamp;
     const uint32_t k0 = (uint32_t)a.sw.seed, k1 = (uint32_t)(a.sw.seed >> 32);
     const double sd = sqrt(a.sw.sigma2);
@@ -571,8 +711,8 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         const uint32_t ubits = (uint32_t)__shfl((int)spare, L.wbase);
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        double lpn = eval_logpsi<T, NM, RAD, AO, WLDS>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
-        if (!(lpn > lp_floor)) lpn = lp_floor;
+        double lpn = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
+        if (lpn < lp_zero) lpn = lp_eps;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
         const double U = u01_closed0(ubits);
         const bool acc =
@@ -632,6 +772,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
     sweep_walkers<T, NM, RAD, AO, true>(a, L, (const T*)qmc_dyn_lds);
+}
+
+template <typename T, int NM, int RAD>
+__global__ __launch_bounds__(64) void sweep_kernel_exc(SArgs a) {
+    const WalkerLane L = lane_layout(a.t, a.nw);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD, AoGeneric, false, true>(a, L);
 }
 
 // Sweeps of many systems (each with its own walkers, step count and seed) in

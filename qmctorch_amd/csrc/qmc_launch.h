@@ -47,6 +47,9 @@ template <int RAD> int launch_sweep_f64(const SArgs& a, hipStream_t s);
 template <int RAD> int launch_grad(const GArgs& a, hipStream_t s);
 template <int RAD> int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
 template <int RAD> int launch_eloc_grad(const EArgs& a, hipStream_t s);
+// excitation (trace-trick) kernels, any precision; register buckets up to 16
+template <int RAD> int launch_wf_exc(const KArgs& a, int precision, int mode, hipStream_t s);
+template <int RAD> int launch_sweep_exc(const SArgs& a, int precision, hipStream_t s);
 // HMC / drift: MC = 1 for single-configuration plans, QMC_MAXCONF otherwise
 template <int RAD, int MC> int launch_hmc_f32(const HArgs& a, hipStream_t s);
 template <int RAD, int MC> int launch_hmc_f64(const HArgs& a, hipStream_t s);

@@ -188,8 +188,10 @@ class SystemTables:
         self.nci = cup.shape[0]
         uu, iu = _unique_rows(cup)
         ud, idn = _unique_rows(cdn)
-        if len(uu) > N.QMC_MAXCONF or len(ud) > N.QMC_MAXCONF:
-            raise NotImplementedError("more than %d unique configurations per spin" % N.QMC_MAXCONF)
+        # beyond QMC_MAXCONF unique configurations per spin the library uses its
+        # excitation (trace-trick) kernels, for single and double excitations of
+        # the ground state (qmc_plan_create checks)
+        self.excitations = len(uu) > N.QMC_MAXCONF or len(ud) > N.QMC_MAXCONF
         self.conf_up = np.ascontiguousarray(uu.reshape(-1)).astype(np.int32)
         self.conf_dn = np.ascontiguousarray(ud.reshape(-1)).astype(np.int32)
         if self.ndown == 0:

@@ -52,6 +52,11 @@ CASES = {
     "h2o_ccpvdz_ee_en": lambda: (systems.h2o(), dict(w_ee=1.0, w_en=1.0)),
     # 4 electrons with MO columns up to 9: register bucket 12 at 16 walkers per wavefront (the
     # larger-bucket elimination with group leaders on all lanes), 64 CI terms, 8 unique per spin
+    # more than 8 unique configurations per spin: the excitation (trace-trick) kernels
+    "lih_631g_sd46_ee_en": lambda: (systems.lih(), dict(configs="single_double(4,6)", w_ee=0.9, w_en=1.1,
+                                                         ci=[1.0] + list(0.15 * np.random.default_rng(8).standard_normal(92)))),
+    "h2o_ccpvdz_sd46_ee": lambda: (systems.h2o(), dict(configs="single_double(4,6)", w_ee=1.0,
+                                                        ci=[1.0] + list(0.1 * np.random.default_rng(9).standard_normal(92)))),
     "lih_631g_cas28_ee": lambda: (systems.lih(), dict(configs="cas(2,8)", w_ee=0.8,
                                                        ci=[1.0] + list(0.2 * np.random.default_rng(7).standard_normal(63)))),
     "n2_ccpvdz_ee": lambda: (systems.n2(), dict(w_ee=0.9)),

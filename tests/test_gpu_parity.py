@@ -188,7 +188,8 @@ def test_sweep_gaussian_moments():
     assert abs(m - 1 / (4 * alpha)) < 6 * err
 
 
-@pytest.mark.parametrize("case", ["lih_631g_cas28_ee", "c6h6_ccpvdz_ee_en"])
+@pytest.mark.parametrize("case", ["lih_631g_cas28_ee", "c6h6_ccpvdz_ee_en", "lih_631g_sd46_ee_en",
+                                  "h2o_ccpvdz_sd46_ee"])
 def test_sweep_logpsi_matches_fp64_evaluation(case):
     """The log|psi| a mixed-precision sweep keeps for its walkers equals an FP64
     evaluation at the final positions (register buckets above 8: the LDS-row
@@ -217,6 +218,28 @@ def test_sweep_logpsi_matches_fp64_evaluation(case):
     assert int(na.sum()) > 0
     assert float(d.max()) < 1e-3 * max(1.0, float(l64.abs().max()) / 50)
     assert torch.equal(sg, s64)
+
+
+def test_sweep_invariant_under_wavefunction_scale():
+    """psi^2 is sampled up to its normalisation (metropolis.py replaces only an
+    exactly zero density by eps): scaling every MO coefficient by 1e-6 shifts
+    log|psi| by a constant and leaves the chain unchanged, also where |psi| is
+    far below 1e-8."""
+    _say("case sweep scale invariance")
+    mol, kw = helpers.CASES["h2o_ccpvdz_ee_en"]()
+    wf, _ = helpers.make_pair(mol, precision="fp64", **kw)
+    wfs, _ = helpers.make_pair(mol, precision="fp64", **kw)
+    with torch.no_grad():
+        wfs.mo.mo_modifier.mul_(1e-6)
+    init = helpers.positions(mol, 240, seed=41).to(DEV)
+    a = _sampler(mol, 240, 30, seed=3)(wf.pdf, pos=init.clone(), with_tqdm=False).detach()
+    b = _sampler(mol, 240, 30, seed=3)(wfs.pdf, pos=init.clone(), with_tqdm=False).detach()
+    la, _ = wfs.log_psi(b, "fp64")
+    same = (a == b).all(dim=1).double().mean().item()
+    _say("  log|psi| of the scaled chain in [%.1f, %.1f], identical walkers %.4f"
+         % (float(la.min()), float(la.max()), same))
+    assert float(la.max()) < -40.0
+    assert same >= 0.99
 
 
 def test_sweep_batch_invariance():
@@ -381,3 +404,17 @@ def test_solver_run_auto_gradient():
     out = solver.run(2)
     assert np.all(np.isfinite(out.energy))
     assert not torch.equal(before, wf.mo.mo_modifier.detach())
+
+
+def test_excitation_plans_refuse_entry_points_without_that_path():
+    """Plans with more than 8 unique configurations per spin run psi, pdf, the
+    local energy and the sweep through the excitation kernels; the adjoints,
+    the drift and the batched launches say that they have no such path."""
+    _say("case excitation refusals")
+    mol, kw = helpers.CASES["lih_631g_sd46_ee_en"]()
+    wf, _ = helpers.make_pair(mol, **kw)
+    pos = helpers.positions(mol, 8, seed=2).to(DEV)
+    for call in (lambda: wf.gradients_jacobi(pos), lambda: wf.drift(pos),
+                 lambda: (wf(pos).sum()).backward()):
+        with pytest.raises(RuntimeError, match="excitation"):
+            call()
