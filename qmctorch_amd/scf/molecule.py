@@ -95,7 +95,10 @@ class Molecule:
 
         if load is not None:
             log.info("  Loading data from {file}", file=load)
-            self._load_npz(load)
+            if str(load).endswith(".npz"):
+                self._load_npz(load)
+            else:
+                self._load_hdf5(load)
             self.hdf5file = load
             return
 
@@ -178,6 +181,26 @@ class Molecule:
             if n > 1:
                 name += str(n)
         return name
+
+    def _load_hdf5(self, filename: str) -> None:
+        """Molecule cache in the reference's HDF5 layout (molecule.py:394-424):
+        group "molecule" loaded attribute by attribute, then the integer and
+        string fields cast back from their stored types."""
+        from ..utils.hdf5_utils import load_from_hdf5
+
+        load_from_hdf5(self, filename, "molecule")
+        self.atoms = np.asarray(self.atoms).astype("U")
+        self.atomic_nelec = [int(z) for z in self.atomic_nelec]
+        self.atomic_number = [int(z) for z in np.asarray(self.atomic_number).ravel()]
+        self.atom_coords = np.asarray(self.atom_coords, dtype=np.float64).tolist()
+        for k in ("nelec", "nup", "ndown", "natom"):
+            setattr(self, k, int(getattr(self, k)))
+        b = self.basis
+        b.nao, b.nmo = int(b.nao), int(b.nmo)
+        for k in ("radial_type", "harmonics_type"):
+            if isinstance(getattr(b, k, None), (bytes, np.bytes_)):
+                setattr(b, k, getattr(b, k).decode("utf-8"))
+        self.mo_source = "file"
 
     def _load_npz(self, filename: str) -> None:
         """Molecule cache converted from the reference's HDF5 layout

@@ -54,8 +54,9 @@ class WaveFunction(torch.nn.Module):
         return sum(p.numel() for p in self.parameters() if p.requires_grad)
 
     def load(self, filename: str, group: Optional[str] = "wf_opt", model: Optional[str] = "best"):
-        """Load trained parameters (wf_base.py:257-277).  Accepts the reference's
-        HDF5 output when h5py is importable, or an .npz/.pt state dict."""
+        """Load trained parameters (wf_base.py:257-277): group/models/<model>
+        of the solver's HDF5 output (read with h5py or, where it is absent,
+        this package's reader), or an .npz/.pt state dict."""
         if filename.endswith(".npz"):
             d = np.load(filename, allow_pickle=False)
             pre = "%s/models/%s/" % (group, model)
@@ -63,9 +64,9 @@ class WaveFunction(torch.nn.Module):
         elif filename.endswith(".pt") or filename.endswith(".pth"):
             data = torch.load(filename, weights_only=True)
         else:
-            import h5py
+            from ..utils.hdf5_utils import open_file
 
-            with h5py.File(filename, "r") as f5:
+            with open_file(filename, "r") as f5:
                 grp = f5[group]["models"][model]
-                data = {name: torch.as_tensor(val[()]) for name, val in grp.items()}
+                data = {name: torch.as_tensor(np.asarray(val[()])) for name, val in grp.items()}
         self.load_state_dict(data)

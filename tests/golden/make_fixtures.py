@@ -14,6 +14,11 @@ is imported or executed.  Outputs (committed, small):
   states dumped by the solver groups and the (pos, local_energy) pairs of the
   single_point groups
 * ``C_cube.npz``      -- tests/cube/C_AO_*.cub and C_MO_*.cub grids (21x21 plane)
+
+Also committed, copied byte for byte from the reference's tests/hdf5 (data,
+not generated): ``LiH_adf_dz.hdf5``, ``H2_adf_dzp.hdf5``, ``C_adf_dzp.hdf5``,
+the HDF5 files the .npz molecule caches above were decoded from; the package's
+own HDF5 reader is checked against both.
 """
 import glob
 import os
