@@ -577,4 +577,113 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
     logdet = singular ? -INFINITY : log_mantissa(mant) + 0.6931471805599453 * (double)expo;
 }
 
+// ------------------------------------------------ augmented elimination (LDS)
+// lane_gauss_jordan<NM, true> with the pivot row passed through LDS: same
+// pivots, same arithmetic, same outputs, for matrices a whose columns
+// n..NM-1 are zero on entry (gathered configurations).
+// rowbuf is the group's n * NM doubles (the lane_det_lds buffer).  Up to
+// NM = 16 the pivot lane stores its b[0..NM) at rowbuf[0..NM) and its a[j],
+// k < j < n, at rowbuf[NM - 1 + j] (at most NM + n - 1 doubles, so it fits
+// for every n <= NM); readers take the a-part only for j < n (the columns
+// past n stay zero, as in the shuffle version, where the pivot row's entries
+// there are zero).  For the larger buckets, where unmasked loads of whole b
+// rows raise the register pressure, b must also be zero past n: the row is
+// b[0..n) at rowbuf[0..n) and a[j] at rowbuf[n - 1 + j], both read for j < n
+// only (at most 2n - 1 doubles).  The loads of the masked columns may read up
+// to 2 NM - 2 doubles past rowbuf, which the buffer's tail padding keeps in
+// bounds.
+template <int NM>
+__device__ __forceinline__ void lane_gj_lds(double (&a)[NM], double (&b)[NM], int n, int nmax, int myrow, int gbase,
+                                            uint64_t gmask, bool may_store, double* rowbuf, double& logdet,
+                                            double& sgn, double& trace_part, int& my_col) {
+    bool done = false;
+    my_col = -1;
+    sgn = 1.0;
+    int inversions = 0;
+    bool singular = false;
+    double mant = 1.0;  // |det| = mant * 2^expo, renormalised after every pivot
+    int expo = 0;
+    constexpr bool kFullB = NM <= 16;
+    double* const ra = rowbuf + ((kFullB ? NM : n) - 1);  // a-part: ra[j] for k < j < n
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        if (k < nmax) {
+            const bool colk = k < n;
+            int br;
+            if constexpr (NM <= 8) {
+                const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+                double c[NM];
+#pragma unroll
+                for (int s = 0; s < NM; ++s) c[s] = __shfl(cand, gbase + (s < n ? s : 0));
+                double bb = -1.0;
+                br = 0;
+#pragma unroll
+                for (int s = 0; s < NM; ++s) {
+                    if (s < n && c[s] > bb) {
+                        bb = c[s];
+                        br = s;
+                    }
+                }
+            } else {
+                double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
+                int brow = myrow;
+#pragma unroll
+                for (int off = 1; off < NM; off <<= 1) {
+                    const double ob = __shfl_down(best, off);
+                    const int orow = __shfl_down(brow, off);
+                    if (myrow + off < n && ob > best) {
+                        best = ob;
+                        brow = orow;
+                    }
+                }
+                br = __shfl(brow, gbase);
+            }
+            const int plane = gbase + br;
+            const uint64_t bal = __ballot(!done && myrow < br);
+            if (colk) inversions += __popcll(bal & gmask);
+            const double piv = __shfl(a[k], plane);
+            const bool is_piv = colk && (myrow == br);
+            if (colk) {
+                singular = singular || (piv == 0.0);
+                int ex;
+                mant = frexp(mant * fabs(piv), &ex);
+                expo += ex;
+                sgn *= (piv < 0.0) ? -1.0 : 1.0;
+            }
+            const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+            const double f = (colk && !is_piv) ? a[k] * ip : 0.0;
+            wave_lds_sync();  // the previous column's reads are done
+            if (is_piv && may_store) {
+#pragma unroll
+                for (int j = 0; j < NM; ++j)
+                    if (kFullB || j < n) rowbuf[j] = b[j];
+#pragma unroll
+                for (int j = k + 1; j < NM; ++j)
+                    if (j < n) ra[j] = a[j];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int j = k + 1; j < NM; ++j) {
+                const double pa = (j < n) ? ra[j] : 0.0;
+                a[j] = is_piv ? a[j] * ip : a[j] - f * pa;
+            }
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                const double pb = (kFullB || j < n) ? rowbuf[j] : 0.0;
+                b[j] = is_piv ? b[j] * ip : b[j] - f * pb;
+            }
+            if (is_piv) {
+                done = true;
+                my_col = k;
+            }
+        }
+    }
+    if (inversions & 1) sgn = -sgn;
+    logdet = singular ? -INFINITY : log_mantissa(mant) + 0.6931471805599453 * (double)expo;
+    trace_part = 0.0;
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+        if (j == my_col) trace_part = b[j];
+}
+
 }  // namespace qmc

@@ -251,14 +251,36 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
     return k;
 }
 
-// Pivot-row buffer of lane_det_lds for the group of this lane: one NM-double
-// row per lane of the wavefront, used by the group whose first row is that
-// lane (gbase, 0..63), so any lane layout fits.  Lanes past the wavefront's
-// last walker compute a throw-away group and do not store.
+// Row buffer of this wavefront: NM doubles per lane (64 * NM), shared by the
+// eliminations and the row transposes of the kernel instance; 2 NM doubles of
+// tail padding keep the masked loads of lane_gj_lds in bounds.
+template <int NM>
+__device__ __forceinline__ double* wave_rowbuf() {
+    __shared__ double rows[kBlockWaves * 64 * NM + 2 * NM];
+    return rows + (threadIdx.x >> 6) * 64 * NM;
+}
+
+// Pivot-row buffer of lane_det_lds / lane_gj_lds for the group of this lane:
+// the NM-double rows of the group's lanes gbase .. gbase + n - 1, so any lane
+// layout fits.  Lanes past the wavefront's last walker compute a throw-away
+// group and do not store.
 template <int NM>
 __device__ __forceinline__ double* det_rowbuf(const WalkerLane& L) {
-    __shared__ double rows[kBlockWaves * 64 * NM];
-    return rows + ((threadIdx.x >> 6) * 64 + L.gbase) * NM;
+    return wave_rowbuf<NM>() + L.gbase * NM;
+}
+
+// Augmented elimination [A | B] of this lane's group: the shuffle version up to
+// 8 rows (faster for 5 x 5 blocks: H2O E_L 32.9 vs 33.9 us, HMC launch 1.10 vs
+// 1.20 ms), the LDS pivot-row version above (the benzene adjoint kernel spills
+// 360 instead of 1145 VGPRs).
+template <int NM>
+__device__ __forceinline__ void lane_gj(const WalkerLane& L, int nmax, double (&a)[NM], double (&b)[NM], double& lg,
+                                        double& sg, double& tp, int& col) {
+    if constexpr (NM <= 8)
+        lane_gauss_jordan<NM, true>(a, b, L.n_sig, nmax, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    else
+        lane_gj_lds<NM>(a, b, L.n_sig, nmax, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg, tp,
+                        col);
 }
 
 // Single configuration, single CI term: log|D_up D_dn| and its sign (ground state).
@@ -330,7 +352,7 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
                              sg);
         } else {
             int col;
-            lane_gauss_jordan<NM, KIN>(a, b, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+            lane_gj<NM>(L, t.nsig_max, a, b, lg, sg, tp, col);
         }
         const double tr = KIN ? seg_sum(tp, L.gbase, L.n_sig, t.nsig_max) : 0.0;
 #pragma unroll
@@ -916,8 +938,8 @@ __device__ __forceinline__ double eval_drift(const Tables& t, const QmcParams& p
         for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
         double lg, sg, tp;
         int col;
-        lane_gauss_jordan<NM, true>(aa, ib, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
-        wave_lds_sync();  // the previous configuration's reads are done
+        lane_gj<NM>(L, t.nsig_max, aa, ib, lg, sg, tp, col);
+        wave_lds_sync();  // earlier reads of the row buffer are done
         if (L.lane_real && col >= 0) {
 #pragma unroll
             for (int j = 0; j < NM; ++j) lds[(L.gbase + col) * NM + j] = ib[j];
@@ -1036,11 +1058,10 @@ struct HArgs {
 };
 
 template <typename T, int NM, int RAD, int MC, typename AO = AoGeneric>
-__global__ __launch_bounds__(kBlock) void hmc_kernel(HArgs a) {
-    __shared__ double lds_block[kBlock * NM];
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 8 ? 2 : 1))) void hmc_kernel(HArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    double* lds = lds_block + (threadIdx.x >> 6) * 64 * NM;
+    double* lds = wave_rowbuf<NM>();  // row transposes; above 8 rows also the pivot rows
     const Tables& t = a.t;
     const int Ne = t.nelec;
     double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
@@ -1204,7 +1225,11 @@ __device__ __forceinline__ void conf_dets(const Tables& t, const WalkerLane& L, 
         for (int j = 0; j < NM; ++j) bb[j] = (j == L.myrow) ? 1.0 : 0.0;
         double lg, sg, tp;
         int col;
-        lane_gauss_jordan<NM, INV>(aa, bb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        if constexpr (INV)
+            lane_gj<NM>(L, t.nsig_max, aa, bb, lg, sg, tp, col);
+        else
+            lane_det_lds<NM>(aa, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg,
+                             sg);
 #pragma unroll
         for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
             if (c2 == c) {
@@ -1588,7 +1613,7 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
         gather_conf<NM>(t, L, c, bk, bb);
         double lg, sg, tp;
         int col;
-        lane_gauss_jordan<NM, true>(aa, bb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        lane_gj<NM>(L, t.nsig_max, aa, bb, lg, sg, tp, col);
         const double tr = seg_sum(tp, L.gbase, L.n_sig, t.nsig_max);
 #pragma unroll
         for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
@@ -1674,11 +1699,11 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
         gather_conf<NM>(t, L, c, bk, xb);
         double lg, sg, tp;
         int col, col2;
-        lane_gauss_jordan<NM, true>(aa, xb, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+        lane_gj<NM>(L, t.nsig_max, aa, xb, lg, sg, tp, col);
         gather_conf<NM>(t, L, c, mo, aa);
 #pragma unroll
         for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
-        lane_gauss_jordan<NM, true>(aa, ib, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col2);
+        lane_gj<NM>(L, t.nsig_max, aa, ib, lg, sg, tp, col2);
         double inv[NM], zz[NM];
         inverse_columns<NM>(L, t.nsig_max, col, xb, ib, inv, zz);
         double rc = 0.0, oc = 0.0;
