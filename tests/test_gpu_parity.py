@@ -316,6 +316,23 @@ def test_solver_single_point_and_run():
     out = solver.run(3)
     assert len(out.energy) == 4 and np.all(np.isfinite(out.energy))
     assert not torch.equal(before, wf.mo.mo_modifier.detach())
+    # the HDF5 output in the reference's layout (solver_base.py:381-385, solver.py:254-264)
+    from qmctorch_amd.utils.hdf5_utils import open_file
+
+    with open_file("/tmp/qmc_test_h2.hdf5", "r") as f5:
+        sp, opt_grp = f5["single_point"], f5["wf_opt"]
+        kind = sp.attrs["type"]
+        assert (kind.decode() if isinstance(kind, bytes) else str(kind)) == "single_point"
+        np.testing.assert_array_equal(sp["local_energy"][()], obs.local_energy.cpu().numpy())
+        np.testing.assert_array_equal(sp["pos"][()], obs.pos.detach().cpu().numpy())
+        assert float(sp["energy"][()]) == float(obs.energy)
+        np.testing.assert_allclose(opt_grp["energy"][()], np.asarray(out.energy, dtype=np.float64), rtol=0, atol=0)
+    last = {k: v.detach().clone() for k, v in wf.state_dict().items()}
+    with torch.no_grad():
+        wf.mo.mo_modifier.add_(1.0)
+    wf.load("/tmp/qmc_test_h2.hdf5", "wf_opt", "last")
+    for k, v in wf.state_dict().items():
+        assert torch.equal(v.cpu(), last[k].cpu()), k
 
 
 @pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22"])
