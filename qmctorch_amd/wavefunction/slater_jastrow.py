@@ -207,6 +207,8 @@ class SlaterJastrow(WaveFunction):
         out = {k: torch.empty(nw, dtype=torch.float64, device=x.device)
                for k in (("psi", "logabs", "sign") if mode == N.QMC_EVAL_PSI
                          else ("psi", "logabs", "sign", "kinetic", "eloc", "epot"))}
+        if nw == 0:  # an empty batch (its tensors have no storage to hand over)
+            return out
         o = N.QmcWfOut()
         for k, v in out.items():
             setattr(o, k, v.data_ptr())
@@ -279,6 +281,8 @@ class SlaterJastrow(WaveFunction):
         dr = torch.empty(nw, self.nelec, 3, dtype=torch.float64, device=x.device)
         la = torch.empty(nw, dtype=torch.float64, device=x.device)
         sg = torch.empty(nw, dtype=torch.float64, device=x.device)
+        if nw == 0:
+            return dr, la, sg
         N.check(N.load().qmc_wf_drift(plan.handle, prm, _PREC[precision], x.data_ptr(), nw, dr.data_ptr(),
                                       la.data_ptr(), sg.data_ptr(), N.current_stream()))
         return dr, la, sg
