@@ -382,6 +382,49 @@ def config_legs(precision="mixed"):
     return out
 
 
+def walker_scan(mol, wf, precision, counts=(16384, 32768, 65536), launches=8, spl=50):
+    """The same sweep kernel at larger walker counts (one launch = spl steps):
+    how far the 8192-walker launch is from filling the chip (1 368 waves on
+    1 024 SIMDs there)."""
+    import numpy as np
+    import torch
+
+    from qmctorch_amd import _native as N
+    from tests import helpers
+
+    F, T = algorithmic_cost(mol, wf)
+    plan = wf.device_plan()
+    prm, keep = wf._params(precision)
+    prec = N.QMC_PREC_MIXED if precision == "mixed" else N.QMC_PREC_FP64
+    lib, stream = N.load(), N.current_stream()
+    rows = []
+    for nw in counts:
+        x = helpers.positions(mol, nw, seed=7).cuda().contiguous()
+        la = torch.empty(nw, dtype=torch.float64, device="cuda")
+        sg = torch.empty_like(la)
+        nacc = torch.zeros(nw, dtype=torch.int32, device="cuda")
+        wk = N.QmcWalkers()
+        wk.pos, wk.logabs, wk.sign, wk.naccept, wk.walker_id = x.data_ptr(), la.data_ptr(), sg.data_ptr(), \
+            nacc.data_ptr(), 0
+        sw = N.QmcSweepDesc()
+        sw.sigma2, sw.seed, sw.keep, sw.keep_every, sw.step_size = 0.2 / (2 * math.sqrt(2 * math.log(2.0))), 99, 0, 1, 0.2
+        times = []
+        for i in range(launches + 2):
+            sw.step0, sw.nstep, sw.init_logpsi = i * spl, spl, 1 if i == 0 else 0
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            N.check(lib.qmc_metropolis_sweep(plan.handle, prm, prec, wk, nw, sw, stream))
+            e1.record()
+            if i >= 2:
+                times.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in times]))
+        tf = (F + 4 * T) * nw * spl / (ms * 1e-3) / 1e12
+        rows.append({"walkers": nw, "launch_ms": ms, "walker_steps_per_s": nw * spl / (ms * 1e-3),
+                     "achieved_tflops": tf, "frac": tf / PEAK_FP32_TFLOPS})
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -396,6 +439,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-system batched-launch leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-loop leg")
     ap.add_argument("--no-hmc", action="store_true", help="skip the Hamiltonian Monte Carlo leg")
+    ap.add_argument("--no-scan", action="store_true", help="skip the larger-walker-count sweep scan")
     ap.add_argument("--serving-rates", default="50,500,2000", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serving-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -540,6 +584,8 @@ def main():
             "flops_per_walker_step": F + 4 * T,
         },
     }
+    if not args.no_scan:
+        res["walker_scan"] = walker_scan(mol, wf, args.precision)
     if not args.no_hmc:
         res["hmc"] = hmc_leg(mol, wf, nw, reference=not args.no_cpu_baseline)
     if not args.no_configs:
