@@ -162,6 +162,9 @@ typedef struct QmcSweepDesc {
     int32_t uniform_move; /* 1: uniform cube proposal step*(2U-1) (metropolis.py:263-269) */
     double step_size;
     double* keep;        /* [nkeep * nw * 3*nelec] or NULL */
+    int32_t one_electron; /* 1: each step moves one electron per walker, chosen uniformly
+                             (move type "one-elec", metropolis.py:228-250); 0: all electrons */
+    int32_t reserved;
 } QmcSweepDesc;
 
 /* Plan lifetime.  Replaces SlaterJastrow.__init__ (slater_jastrow.py:31-117)

@@ -73,7 +73,7 @@ class QmcSweepDesc(ctypes.Structure):
         ("step0", ctypes.c_int64), ("nstep", ctypes.c_int32), ("init_logpsi", ctypes.c_int32),
         ("sigma2", ctypes.c_double), ("seed", ctypes.c_uint64), ("keep_from", ctypes.c_int64),
         ("keep_every", ctypes.c_int32), ("uniform_move", ctypes.c_int32), ("step_size", ctypes.c_double),
-        ("keep", ctypes.c_void_p),
+        ("keep", ctypes.c_void_p), ("one_electron", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 

@@ -731,6 +731,14 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
                                    ? rnd.w
                                    : ((rnd.x & 0xFFu) | ((rnd.y & 0xFFu) << 8) | ((rnd.z & 0xFFu) << 16) | (rnd.w << 24));
         const uint32_t ubits = (uint32_t)__shfl((int)spare, L.wbase);
+        if (a.sw.one_electron) {
+            // move type "one-elec" (metropolis.py:228-250): only electron
+            // floor(Ne V / 2^32) of the walker moves, V the spare bits of
+            // electron 1's draw (for Ne = 1 the index is 0 whatever V is)
+            const uint32_t vb = (uint32_t)__shfl((int)spare, L.wbase + 1);
+            const int moved = (int)(((uint64_t)vb * (uint32_t)Ne) >> 32);
+            if (L.e != moved) nx = ny = nz = 0.0;
+        }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
         double lpn = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);

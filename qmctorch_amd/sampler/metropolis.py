@@ -1,7 +1,9 @@
 """Metropolis sampler driving the persistent HIP sweep (sampler/metropolis.py:10-298).
 
 Same constructor, ``__call__(pdf, pos=None, with_tqdm=True)`` contract and
-semantics as the reference: all-electron Gaussian proposals with covariance
+semantics as the reference: all-electron ("all-elec") or one-electron
+("one-elec": one uniformly chosen electron per walker and step) Gaussian
+proposals with covariance
 sigma = step/(2 sqrt(2 ln 2)) per coordinate (metropolis.py:206-212; the
 per-coordinate standard deviation is sqrt(sigma)), uniform-cube proposals
 step*(2U-1), acceptance min(1, psi'^2/psi^2) - U >= 0, psi = 0 treated as
@@ -12,7 +14,8 @@ Differences forced by the device design: the chain runs inside
 from Philox4x32-10 keyed by a seed drawn from torch's global generator, so
 ``torch.manual_seed`` still makes runs reproducible, but the streams differ
 from the reference's host MultivariateNormal draws (chains agree
-statistically, not sample by sample).
+statistically, not sample by sample).  "all-elec-iter" (one accept/reject
+per electron in turn) is not implemented and raises.
 """
 import math
 from time import time
@@ -72,9 +75,9 @@ class Metropolis(SamplerBase):
         if not isinstance(wf, SlaterJastrow) or getattr(pdf, "__name__", "") != "pdf":
             raise NotImplementedError("Metropolis samples SlaterJastrow.pdf with the fused HIP sweep; "
                                       "pass wf.pdf of a qmctorch_amd SlaterJastrow")
-        if self.movedict["type"] != "all-elec":
-            raise NotImplementedError("move type %s: the device sweep implements all-electron moves"
-                                      % self.movedict["type"])
+        if self.movedict["type"] == "all-elec-iter":
+            raise NotImplementedError("move type all-elec-iter: the device sweep implements 'all-elec' and "
+                                      "'one-elec' moves")
         if self.ntherm >= self.nstep:
             raise ValueError("Thermalisation longer than trajectory")
         if self.ntherm < 0:
@@ -105,6 +108,7 @@ class Metropolis(SamplerBase):
         sw.keep_from = self.ntherm
         sw.keep_every = self.ndecor
         sw.uniform_move = 1 if self.movedict["proba"] == "uniform" else 0
+        sw.one_electron = 1 if self.movedict["type"] == "one-elec" and self.nelec > 1 else 0
         sw.step_size = self.step_size
         sw.keep = keep.data_ptr()
         lib = N.load()
