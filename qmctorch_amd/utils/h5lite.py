@@ -48,7 +48,7 @@ class Dataset:
 
     def __getitem__(self, key):
         d = self._data
-        if key == () or key is Ellipsis:
+        if key is Ellipsis or (isinstance(key, tuple) and len(key) == 0):
             if isinstance(d, np.ndarray) and d.ndim == 0:
                 return d[()]
             return d
