@@ -258,15 +258,22 @@ class _Reader:
         if self.b[p] != 1:
             raise H5Error("object header version %d is not supported" % self.b[p])
         blocks = [(p + 16, self.u(p + 8, 4))]
+        seen = {p + 16}
         out = []
         while blocks:
             start, size = blocks.pop(0)
+            if start + size > len(self.b):
+                raise H5Error("object header block past the end of the file")
             q, end = start, start + size
             while q + 8 <= end:
                 mtype, msize, flags = self.u(q, 2), self.u(q + 2, 2), self.b[q + 4]
                 d = q + 8
                 if mtype == 0x10:  # continuation
-                    blocks.append((self.at(self.u(d, 8)), self.u(d + 8, 8)))
+                    nxt = self.at(self.u(d, 8))
+                    if nxt in seen:
+                        raise H5Error("object header continuation cycle")
+                    seen.add(nxt)
+                    blocks.append((nxt, self.u(d + 8, 8)))
                 elif mtype != 0:
                     if flags & 0x02:
                         raise H5Error("shared header messages are not supported")
@@ -328,9 +335,13 @@ class _Reader:
             raise H5Error("bad local heap signature")
         data = self.at(self.u(hp + 24, 8))
         out = []
+        visited = set()
 
         def node(a):
             p = self.at(a)
+            if p in visited:
+                raise H5Error("group B-tree cycle")
+            visited.add(p)
             if self.b[p:p + 4] != b"TREE" or self.b[p + 4] != 0:
                 raise H5Error("bad group B-tree node")
             level, n = self.b[p + 5], self.u(p + 6, 2)

@@ -235,3 +235,17 @@ def test_wavefunction_load_from_solver_output(tmp_path):
     wf.load(p, "wf_opt", "best")
     for k, v in wf.state_dict().items():
         torch.testing.assert_close(v, best[k], rtol=0, atol=0)
+
+
+def test_corrupt_or_foreign_files_raise(tmp_path):
+    p = tmp_path / "not.h5"
+    p.write_bytes(b"this is not an HDF5 file at all" * 4)
+    with pytest.raises(h5lite.H5Error):
+        h5lite.File(str(p), "r")
+    good = open(os.path.join(GOLDEN, "LiH_adf_dz.hdf5"), "rb").read()
+    t = tmp_path / "trunc.h5"
+    t.write_bytes(good[:4096])  # object headers and heaps past the cut
+    with pytest.raises((h5lite.H5Error, IndexError, ValueError, KeyError)):
+        h5lite.File(str(t), "r")
+    with pytest.raises(FileNotFoundError):
+        h5lite.File(str(tmp_path / "missing.h5"), "r")
