@@ -828,9 +828,14 @@ PARITY = {
     "pool/operator": "pinned: LAPACK det/inverse vs Leibniz on 2x2/3x3, Jacobi vs autograd",
     "JastrowEE/JastrowEN/combine_jastrow": "pinned: autograd gradient/Laplacian, cusp slope, permutation invariance",
     "kinetic_energy_jacobi/local_energy": "pinned: autograd Laplacian, H-atom and single-Gaussian closed forms, LiH golden fixture",
-    "metropolis": "pinned statistically: exact eigenfunction (zero variance), detailed-balance moments",
+    "metropolis": "pinned statistically (tests/test_oracle_sampler_gradient.py): hydrogen 1s E_L = -1/2 on every "
+                  "kept sample and <r> = 3/2; Gaussian <x^2> = 1/(4 alpha) for normal and uniform moves; first-step "
+                  "acceptance (1 + sigma)^(-3/2) from the maximum of exp(-|x|^2/2); flat-density proposal variance "
+                  "sigma and step^2/3; kept counts of ntherm / ndecor",
     "hmc_step/hamiltonian": "pinned: leapfrog of a Gaussian target against the closed-form linear map, exact "
                             "energy change, Gaussian moments (tests/test_oracle_hmc.py)",
     "grad_potential": "pinned: -2 grad ln|psi| against gradients_jacobi / psi and central differences",
-    "manual_gradient": "pinned: finite differences of <E_L> weights",
+    "manual_gradient": "pinned (tests/test_oracle_sampler_gradient.py): Richardson-extrapolated central "
+                       "differences (h = 1e-5) of the reweighted energy sum r E_L,0 / sum r, r = psi^2/psi_0^2, on "
+                       "fixed positions, for mo_modifier, both Pade weights and CI weights (H2O, LiH CAS(2,2))",
 }

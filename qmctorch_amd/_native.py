@@ -13,6 +13,10 @@ LIBNAME = "libqmcmi355x.so"
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, LIBNAME)
 
+QMC_OK = 0
+QMC_ERR_ARG = 1
+QMC_ERR_HIP = 2
+QMC_ERR_UNSUPPORTED = 3
 QMC_MAXP = 12
 QMC_MAXCONF = 8
 QMC_RADIAL_GTO_PURE = 0

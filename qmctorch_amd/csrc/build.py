@@ -1,6 +1,6 @@
 """Build libqmcmi355x.so (HIP, gfx950) in-tree.
 
-    python -m qmctorch_amd.csrc.build [--verbose] [--resource-usage]
+    python -m qmctorch_amd.csrc.build [--verbose] [--resource-usage] [--incremental]
 
 Each kernel family is its own translation unit, compiled in parallel and
 linked into qmctorch_amd/libqmcmi355x.so (next to the package, so the library
@@ -66,12 +66,26 @@ def spec_units():
     return [(f, "spec_" + os.path.basename(f)[:-4], []) for f in files]
 
 
-def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) -> str:
+def _stale(unit, deps_mtime: float) -> bool:
+    obj = os.path.join(_obj_dir(), unit[1] + ".o")
+    src = unit[0] if os.path.isabs(unit[0]) else os.path.join(HERE, unit[0])
+    return not os.path.isfile(obj) or os.path.getmtime(obj) < max(deps_mtime, os.path.getmtime(src))
+
+
+def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0, incremental: bool = False) -> str:
+    """Compile every unit and link the library.  With incremental=True only the
+    units older than their source or any header of csrc/ and include/ are
+    recompiled (a developer shortcut; the default always compiles everything)."""
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
     units = UNITS + spec_units()
+    if incremental:
+        hdrs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith(".h")]
+        hdrs += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+        deps = max(os.path.getmtime(h) for h in hdrs)
+        units = [u for u in units if _stale(u, deps)]
     # the largest units first, so the parallel build ends sooner
     units = sorted(units, key=lambda u: 0 if u[0].startswith(os.path.join(_obj_dir(), "spec")) else 1)
-    jobs = jobs or min(len(units), os.cpu_count() or 4, 16)
+    jobs = jobs or max(1, min(len(units), os.cpu_count() or 4, 16))
     with ThreadPoolExecutor(jobs) as ex:
         results = list(ex.map(lambda u: compile_one(u, extra), units))
     for unit, r in zip(units, results):
@@ -80,7 +94,7 @@ def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0) ->
             raise RuntimeError("hipcc failed on %s" % unit[1])
         if verbose or resource_usage:
             sys.stderr.write(r.stderr)
-    objs = [os.path.join(_obj_dir(), u[1] + ".o") for u in units]
+    objs = [os.path.join(_obj_dir(), u[1] + ".o") for u in UNITS + spec_units()]
     tmp = LIB + ".tmp%d" % os.getpid()
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -96,5 +110,6 @@ if __name__ == "__main__":
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--resource-usage", action="store_true")
     ap.add_argument("-j", type=int, default=0)
+    ap.add_argument("--incremental", action="store_true", help="recompile only out-of-date units")
     a = ap.parse_args()
-    print(build(a.verbose, a.resource_usage, a.j))
+    print(build(a.verbose, a.resource_usage, a.j, a.incremental))

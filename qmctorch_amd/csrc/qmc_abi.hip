@@ -110,6 +110,61 @@ uint64_t layout_signature(const QmcSystemDesc* d) {
 }  // namespace
 
 namespace {
+// Range check of every count and index table of a system description: the
+// kernels index registers and device tables with these values unchecked.
+// Returns the reason of the first violation, or nullptr.
+const char* invalid_tables(const QmcSystemDesc* d) {
+    if (d->natom < 1 || d->nao < 1 || d->nprim_ref < 1 || d->ngroup < 1 || d->nentry < 1 || d->nmo < 1 ||
+        d->nci < 1)
+        return "natom, nao, nprim_ref, ngroup, nentry, nmo and nci must be positive";
+    if (d->nmo < d->ndown) return "nmo smaller than ndown";
+    if (!d->grp_atom || !d->grp_nprim || !d->grp_prim0 || !d->grp_prim_src || !d->grp_ao0 || !d->grp_nao ||
+        !d->gao_mu || !d->gao_k || !d->gao_K || !d->gao_n || !d->gao_src || !d->atom_z || !d->jastrow_atoms ||
+        !d->conf_up || !d->ci_up || !d->ci_dn || (d->ndown > 0 && !d->conf_dn))
+        return "missing table pointer";
+    std::vector<int> entry_group(d->nentry, -1);
+    for (int g = 0; g < d->ngroup; ++g) {
+        if (d->grp_atom[g] < 0 || d->grp_atom[g] >= d->natom) return "grp_atom out of range";
+        const int np = d->grp_nprim[g];
+        if (np < 1 || np > QMC_MAXP) return "grp_nprim out of range";
+        if (d->grp_prim0[g] < 0) return "negative grp_prim0";
+        for (int q = 0; q < np; ++q) {
+            const int32_t s = d->grp_prim_src[d->grp_prim0[g] + q];
+            if (s < 0 || s >= d->nprim_ref) return "grp_prim_src out of range";
+        }
+        if (d->grp_ao0[g] < 0 || d->grp_nao[g] < 1 || (int64_t)d->grp_ao0[g] + d->grp_nao[g] > d->nentry)
+            return "AO entries of a group out of range";
+        for (int i = d->grp_ao0[g]; i < d->grp_ao0[g] + d->grp_nao[g]; ++i) {
+            if (entry_group[i] >= 0) return "an AO entry belongs to two groups";
+            entry_group[i] = g;
+        }
+    }
+    for (int i = 0; i < d->nentry; ++i) {
+        if (entry_group[i] < 0) return "an AO entry belongs to no group";
+        if (d->gao_mu[i] < 0 || d->gao_mu[i] >= d->nao) return "gao_mu out of range";
+        const int k = d->gao_k[i];
+        if (k < 0 || (k & 0xF) > 3 || ((k >> 4) & 0xF) > 3 || ((k >> 8) & 0xF) > 3 || (k >> 12) != 0)
+            return "gao_k: Cartesian powers must be in [0, 3]";
+        for (int q = 0; q < QMC_MAXP; ++q) {
+            const int32_t s = d->gao_src[(size_t)i * QMC_MAXP + q];
+            if (s < 0 || s >= d->nprim_ref) return "gao_src out of range";
+        }
+    }
+    for (int j = 0; j < d->nconf_up * d->nup; ++j)
+        if (d->conf_up[j] < 0 || d->conf_up[j] >= d->nmo) return "conf_up column out of range";
+    if (d->ndown > 0)
+        for (int j = 0; j < d->nconf_dn * d->ndown; ++j)
+            if (d->conf_dn[j] < 0 || d->conf_dn[j] >= d->nmo) return "conf_dn column out of range";
+    for (int n = 0; n < d->nci; ++n) {
+        if (d->ci_up[n] < 0 || d->ci_up[n] >= d->nconf_up) return "ci_up out of range";
+        if (d->ci_dn[n] < 0 || d->ci_dn[n] >= d->nconf_dn) return "ci_dn out of range";
+    }
+    if (d->grp_tight)
+        for (int g = 0; g < d->ngroup; ++g)
+            if (d->grp_tight[g] != 0 && d->grp_tight[g] != 1) return "grp_tight must be 0 or 1";
+    return nullptr;
+}
+
 // Excitation table of nconf configurations of n columns each (row c of conf
 // lists its MO columns in the order of the determinant): per configuration
 // {level, h1, h2, p1, p2, sign} with the holes h (ground-state columns 0..n-1
@@ -200,6 +255,8 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         return fail(QMC_ERR_UNSUPPORTED, "radial type must be gto_pure or sto");
     if (d->nconf_up < 1 || d->nconf_up > QMC_MAXEXC || d->nconf_dn < 1 || d->nconf_dn > QMC_MAXEXC)
         return fail(QMC_ERR_UNSUPPORTED, "unique configurations per spin must be in [1, 1024]");
+    // every index table is range-checked before anything reaches the device
+    if (const char* why = invalid_tables(d)) return fail(QMC_ERR_ARG, why);
     // beyond QMC_MAXCONF per spin: single and double excitations of the ground
     // state through the trace-trick kernels
     const bool exc = d->nconf_up > QMC_MAXCONF || d->nconf_dn > QMC_MAXCONF;

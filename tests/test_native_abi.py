@@ -129,3 +129,70 @@ def test_acceptance_bits_uniform_and_independent():
     for k in range(4):
         hi = (words[:, k] >> 8).astype(np.float64) * 2.0 ** -24
         assert abs(np.corrcoef(u, hi)[0, 1]) < 4.0 / np.sqrt(n)
+
+
+def _lih_tables():
+    import numpy as np
+
+    from qmctorch_amd.wavefunction.plan import SystemTables, atomic_orbital_norm
+    from tests import systems
+
+    mol = systems.lih()
+    b = mol.basis
+    cfg = (np.array([[0, 1], [0, 2]]), np.array([[0, 1], [0, 1]]))
+    return SystemTables(b, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown, cfg, 3, True, True,
+                        atomic_orbital_norm(b), np.asarray(b.bas_exp, dtype=np.float64))
+
+
+def _corrupt(tab, field, index, value):
+    arr = getattr(tab, field).copy()
+    arr.reshape(-1)[index] = value
+    setattr(tab, field, arr)
+
+
+@pytest.mark.parametrize("field,index,value,reason", [
+    ("grp_atom", 0, 2, "grp_atom"),
+    ("grp_atom", 1, -1, "grp_atom"),
+    ("grp_prim_src", 3, 10 ** 6, "grp_prim_src"),
+    ("grp_nao", 0, 10 ** 4, "AO entries"),
+    ("grp_ao0", 1, -3, "AO entries"),
+    ("gao_mu", 2, 11, "gao_mu"),
+    ("gao_src", 5, -1, "gao_src"),
+    ("gao_k", 0, 4, "gao_k"),
+    ("conf_up", 3, 11, "conf_up"),
+    ("conf_dn", 0, -1, "conf_dn"),
+    ("ci_up", 1, 2, "ci_up"),
+    ("ci_dn", 0, 1, "ci_dn"),
+])
+def test_plan_create_rejects_out_of_range_tables(field, index, value, reason):
+    """qmc_plan_create range-checks every index table on the host and returns
+    QMC_ERR_ARG before it allocates or copies anything on the device."""
+    if not os.path.isfile(N.LIBPATH):
+        pytest.skip("libqmcmi355x.so not built")
+    lib = N.load()
+    tab = _lih_tables()
+    _corrupt(tab, field, index, value)
+    h = ctypes.c_void_p()
+    desc = tab.desc()
+    assert lib.qmc_plan_create(ctypes.byref(desc), ctypes.byref(h)) == N.QMC_ERR_ARG
+    assert not h.value
+    assert reason in lib.qmc_last_error().decode()
+
+
+def test_plan_create_rejects_bad_counts_and_missing_tables():
+    if not os.path.isfile(N.LIBPATH):
+        pytest.skip("libqmcmi355x.so not built")
+    lib = N.load()
+    tab = _lih_tables()
+    h = ctypes.c_void_p()
+    d = tab.desc()
+    d.nci = 0
+    assert lib.qmc_plan_create(ctypes.byref(d), ctypes.byref(h)) == N.QMC_ERR_ARG
+    d = tab.desc()
+    d.gao_src = None
+    assert lib.qmc_plan_create(ctypes.byref(d), ctypes.byref(h)) == N.QMC_ERR_ARG
+    assert "missing" in lib.qmc_last_error().decode()
+    d = tab.desc()
+    d.nentry = d.nentry - 1  # the last group's entries then run past nentry
+    assert lib.qmc_plan_create(ctypes.byref(d), ctypes.byref(h)) == N.QMC_ERR_ARG
+    assert not h.value
