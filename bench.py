@@ -11,14 +11,15 @@ initialised around the nuclei.  MO coefficients from the package's own RHF
 (no SCF program offline) -- the cost of a step does not depend on their values.
 
 Prints one JSON line (rank 0).  Besides the metric it reports the local-energy
-throughput, the local-energy error of the mixed-precision path against the
-FP64 path on the sampled positions, the roofline position of the sweep kernel
+throughput (FP64), the effect of the mixed-precision sweep on the sampled
+chains against the FP64 sweep, the roofline position of the sweep kernel
 (HIP events on the launch stream) and two baselines of the oracle restatement
 of the reference algorithm: on the host cores (bounded sample) and through
 torch-ROCm on the same GPU at the same walker count (the reference's own
 GPU-PyTorch path).
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -48,10 +49,37 @@ def algorithmic_cost(mol, wf):
 
 
 def build_system(name):
-    from tests import systems
+    from qmctorch_amd.serving import catalog
 
-    return {"h2o": systems.h2o, "lih": systems.lih, "h2": systems.h2, "n2": systems.n2,
-            "c6h6": systems.benzene}[name]()
+    return catalog.molecule(name)
+
+
+def sj_wavefunction(mol, configs="ground_state", w_ee=1.0, w_en=1.0, ci=None, precision="mixed"):
+    """SlaterJastrow through the package's public API (the reference's module
+    paths): Pade e-e (and e-n) Jastrow, HF orbitals of the molecule, CI weights."""
+    import torch
+
+    from qmctorch_amd.wavefunction import SlaterJastrow
+    from qmctorch_amd.wavefunction.jastrows.elec_elec import JastrowFactor as JEE
+    from qmctorch_amd.wavefunction.jastrows.elec_elec import PadeJastrowKernel as PEE
+    from qmctorch_amd.wavefunction.jastrows.elec_nuclei import JastrowFactor as JEN
+    from qmctorch_amd.wavefunction.jastrows.elec_nuclei import PadeJastrowKernel as PEN
+
+    jas = [JEE(mol, PEE, kernel_kwargs={"w": w_ee}, cuda=True)]
+    if w_en is not None:
+        jas.append(JEN(mol, PEN, kernel_kwargs={"w": w_en}, cuda=True))
+    wf = SlaterJastrow(mol, jastrow=jas, configs=configs, cuda=True, precision=precision)
+    if ci is not None:
+        with torch.no_grad():
+            wf.fc.weight.copy_(torch.as_tensor(ci, dtype=torch.float64).view(1, -1))
+    return wf
+
+
+def start_positions(mol, nw, seed):
+    """Walker starts from the package's device initialiser (atomic domain)."""
+    from qmctorch_amd.sampler.walkers import Walkers
+
+    return Walkers(nw, mol.nelec, cuda=True).draw(mol.domain("atomic"), nw, mol.nelec, seed=seed).contiguous()
 
 
 def _restatement(mol, wf_dev):
@@ -65,14 +93,18 @@ def _restatement(mol, wf_dev):
 
 def measured_traffic(args, nw, steps_per_launch):
     """HBM bytes per sweep launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes (scaled by a plain copy of known size), when they were
-    taken on this workload; None otherwise."""
+    WRITE_SIZE passes (scaled by a plain copy of known size) on this workload
+    and launch length; None when no pass was taken for them."""
     path = os.path.join(ROOT, "profiles", "sweep_kernel_%s_%d_traffic.json" % (args.system, nw))
-    if not os.path.isfile(path) or args.precision != "mixed" or steps_per_launch != 50:
+    if not os.path.isfile(path) or args.precision != "mixed":
         return None, None
     with open(path) as f:
         d = json.load(f)
-    return d["traffic_bytes_per_launch"], os.path.relpath(path, ROOT)
+    per = d.get("traffic_bytes_per_launch_by_steps", {})
+    v = per.get(str(steps_per_launch))
+    if v is None and steps_per_launch == d.get("steps_per_launch", 50):
+        v = d.get("traffic_bytes_per_launch")
+    return v, (os.path.relpath(path, ROOT) if v is not None else None)
 
 
 def cpu_baseline(mol, wf_dev, sample_walkers=1024, min_seconds=12.0, max_steps=3000, warm=2):
@@ -82,12 +114,10 @@ def cpu_baseline(mol, wf_dev, sample_walkers=1024, min_seconds=12.0, max_steps=3
     reference's own GPU-PyTorch path (gpu_torch_baseline)."""
     import torch
 
-    from tests import helpers
-
     O, orc = _restatement(mol, wf_dev)
     cores = len(os.sched_getaffinity(0))
     torch.set_num_threads(cores)
-    pos = helpers.positions(mol, sample_walkers, seed=3)
+    pos = start_positions(mol, sample_walkers, seed=3).cpu()
     O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0)
     # chunks of 50 steps until at least min_seconds of CPU work
     sample_steps, dt = 0, 0.0
@@ -110,11 +140,9 @@ def gpu_torch_baseline(mol, wf_dev, nw, sample_steps=200, warm=3):
     workload): the reference's GPU-PyTorch walker-steps/s on this GPU."""
     import torch
 
-    from tests import helpers
-
     O, orc = _restatement(mol, wf_dev)
     orc.to_device("cuda")
-    pos = helpers.positions(mol, nw, seed=3).cuda()
+    pos = start_positions(mol, nw, seed=3)
     O.metropolis(orc.pdf, pos.clone(), warm, step_size=0.2, seed=0, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -135,10 +163,9 @@ def hmc_leg(mol, wf, nw, steps=200, L=10, eps=0.05, ref_steps=3, reference=True)
     import torch
 
     from qmctorch_amd import _native as N
-    from tests import helpers
 
     dev = torch.device("cuda")
-    x = helpers.positions(mol, nw, seed=4).to(dev).contiguous()
+    x = start_positions(mol, nw, seed=4)
     logabs = torch.empty(nw, dtype=torch.float64, device=dev)
     sign = torch.empty_like(logabs)
     nacc = torch.zeros(nw, dtype=torch.int32, device=dev)
@@ -182,7 +209,7 @@ def hmc_leg(mol, wf, nw, steps=200, L=10, eps=0.05, ref_steps=3, reference=True)
     # baseline: the reference algorithm on torch-ROCm (FP64), same walkers, a few steps
     O, orc = _restatement(mol, wf)
     orc.to_device("cuda")
-    pos = helpers.positions(mol, nw, seed=4).cuda()
+    pos = start_positions(mol, nw, seed=4)
     O.hamiltonian(orc.pdf, pos.clone(), 1, step_size=eps, L=L, seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -301,12 +328,10 @@ def config_legs(precision="mixed"):
     from qmctorch_amd.sampler import Metropolis
     from qmctorch_amd.serving import catalog
     from qmctorch_amd.solver import Solver
-    from tests import helpers, systems
-
     out = {}
     # H2/STO-3G FP64, e-e Pade, 1000 walkers x 2000 steps, then E_L of the final positions
-    mol = systems.h2()
-    wf, _ = helpers.make_pair(mol, w_ee=1.0, perturb=0.0, precision="fp64")
+    mol = build_system("h2")
+    wf = sj_wavefunction(mol, w_ee=1.0, w_en=None, precision="fp64")
     pos0 = catalog.initial_positions(mol, 1000, seed=0)
     sampler = Metropolis(nwalkers=1000, nstep=2000, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
                          move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=0)
@@ -325,26 +350,29 @@ def config_legs(precision="mixed"):
                             "local_energy_s": t2 - t1, "energy": float(e.mean()),
                             "error": float(e.std() / np.sqrt(e.numel())), "acceptance": sampler.last_acceptance}
 
-    # LiH/6-31G CAS(2,2), CI [1, N(0, 0.1) x 3], e-e + e-n Pade: mixed vs FP64 local energy on 10^5
-    # Gaussian positions (sigma 1 bohr around the nuclei, seed 1) and 10^5 positions of an FP64 chain
-    mol = systems.lih()
+    # LiH/6-31G CAS(2,2), CI [1, N(0, 0.1) x 3], e-e + e-n Pade.  Local energies are FP64 in both
+    # precisions (error 0); the mixed path's error enters through log|psi| of the sweep: its
+    # difference from FP64 on 10^5 Gaussian positions (sigma 1 bohr around the nuclei, seed 1) and
+    # on 10^5 positions of an FP64 chain, and the acceptance decisions it flips along a chain
+    mol = build_system("lih")
     rng = np.random.default_rng(0)
     ci = [1.0] + list(0.1 * rng.standard_normal(3))
-    wf64, _ = helpers.make_pair(mol, configs="cas(2,2)", w_ee=1.0, w_en=1.0, ci=ci, perturb=0.0, precision="fp64")
-    wfmx, _ = helpers.make_pair(mol, configs="cas(2,2)", w_ee=1.0, w_en=1.0, ci=ci, perturb=0.0, precision="mixed")
+    wf64 = sj_wavefunction(mol, configs="cas(2,2)", ci=ci, precision="fp64")
+    wfmx = sj_wavefunction(mol, configs="cas(2,2)", ci=ci, precision="mixed")
     gauss = catalog.initial_positions(mol, 100000, seed=1)
     chain = Metropolis(nwalkers=10000, nstep=510, ntherm=500, step_size=0.2, nelec=mol.nelec,
                        init=mol.domain("atomic"), move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=1)
     cpos = chain(wf64.pdf, pos=catalog.initial_positions(mol, 10000, seed=2), with_tqdm=False).detach()
     errs = {}
     for name, p in (("gaussian", gauss), ("fp64_chain", cpos)):
-        e64 = wf64.local_energy_parts(p, "fp64")["eloc"]
-        emx = wfmx.local_energy_parts(p, "mixed")["eloc"]
-        d = (emx - e64).abs() / mol.nelec
+        l64, s64 = wf64.log_psi(p, "fp64")
+        lmx, smx = wfmx.log_psi(p, "mixed")
+        d = (lmx - l64).abs()
         ok = torch.isfinite(d)
-        errs[name] = {"positions": int(p.shape[0]), "max": float(d[ok].max()), "mean": float(d[ok].mean()),
-                      "median": float(d[ok].median()),
-                      "bias_of_mean": float((emx[ok].mean() - e64[ok].mean()).abs() / mol.nelec)}
+        errs[name] = {"positions": int(p.shape[0]), "max_abs_dlogpsi": float(d[ok].max()),
+                      "mean_abs_dlogpsi": float(d[ok].mean()), "sign_flips": int((smx != s64).sum())}
+    errs["local_energy_error_per_electron"] = 0.0
+    errs["chain"] = sweep_precision_effect(mol, wfmx, 4096, steps=1000)
     nw = 16384
     st = catalog.initial_positions(mol, nw, seed=3)
     sm = Metropolis(nwalkers=nw, nstep=500, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
@@ -355,12 +383,12 @@ def config_legs(precision="mixed"):
     sm(wfmx.pdf, pos=st, with_tqdm=False)
     torch.cuda.synchronize()
     out["lih_631g_cas22_mixed"] = {"walkers": nw, "walker_steps_per_s": nw * 500 / (time.perf_counter() - t0),
-                                   "eloc_error_per_electron_vs_fp64": errs}
+                                   "mixed_vs_fp64": errs}
 
     # H2O/cc-pVDZ, 8192 walkers, energy minimisation (manual gradient through the adjoint kernel,
     # freeze ao and ci, Adam lr 1e-3, 25 resampling steps per epoch)
-    mol = systems.h2o()
-    wf, _ = helpers.make_pair(mol, w_ee=1.0, w_en=1.0, perturb=0.0, precision=precision)
+    mol = build_system("h2o")
+    wf = sj_wavefunction(mol, precision=precision)
     samp = Metropolis(nwalkers=8192, nstep=100, step_size=0.2, nelec=mol.nelec, init=mol.domain("atomic"),
                       move={"type": "all-elec", "proba": "normal"}, cuda=True, seed=4)
     samp.walkers.initialize(pos=catalog.initial_positions(mol, 8192, seed=4))
@@ -390,7 +418,6 @@ def walker_scan(mol, wf, precision, counts=(16384, 32768, 65536), launches=8, sp
     import torch
 
     from qmctorch_amd import _native as N
-    from tests import helpers
 
     F, T = algorithmic_cost(mol, wf)
     plan = wf.device_plan()
@@ -399,7 +426,7 @@ def walker_scan(mol, wf, precision, counts=(16384, 32768, 65536), launches=8, sp
     lib, stream = N.load(), N.current_stream()
     rows = []
     for nw in counts:
-        x = helpers.positions(mol, nw, seed=7).cuda().contiguous()
+        x = start_positions(mol, nw, seed=7)
         la = torch.empty(nw, dtype=torch.float64, device="cuda")
         sg = torch.empty_like(la)
         nacc = torch.zeros(nw, dtype=torch.int32, device="cuda")
@@ -455,14 +482,13 @@ def main():
     import torch
 
     from qmctorch_amd import _native as N
-    from tests import helpers
 
     torch.set_default_dtype(torch.float64)
     mol = build_system(args.system)
-    wf, _ = helpers.make_pair(mol, w_ee=1.0, w_en=1.0, perturb=0.0, precision=args.precision)
+    wf = sj_wavefunction(mol, precision=args.precision)
     nw = args.walkers
     dev = torch.device("cuda")
-    x = helpers.positions(mol, nw, seed=0).to(dev).contiguous()
+    x = start_positions(mol, nw, seed=0)
     logabs = torch.empty(nw, dtype=torch.float64, device=dev)
     sign = torch.empty(nw, dtype=torch.float64, device=dev)
     nacc = torch.zeros(nw, dtype=torch.int32, device=dev)
@@ -481,43 +507,72 @@ def main():
     sw.step_size = 0.2
     stream = N.current_stream()
     spl = args.steps_per_launch
+    # launch arguments built once: the timed region is ctypes calls of the sweep only
+    sweep_fn = lib.qmc_metropolis_sweep
+    plan_h, prm_ref, wk_ref, sw_ref = plan.handle, ctypes.byref(prm), ctypes.byref(wk), ctypes.byref(sw)
 
-    def run(step0, nstep, init, events=None):
-        s = step0
-        first = init
+    def launches(step0, nstep):
+        out, s = [], step0
         while s < step0 + nstep:
-            sw.step0 = s
-            sw.nstep = min(spl, step0 + nstep - s)
-            sw.init_logpsi = 1 if first else 0
-            if events is not None:
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-            N.check(lib.qmc_metropolis_sweep(plan.handle, prm, prec, wk, nw, sw, stream))
-            if events is not None:
-                e1.record()
-                events.append((e0, e1, sw.nstep))
-            first = False
-            s += sw.nstep
+            n = min(spl, step0 + nstep - s)
+            out.append((s, n))
+            s += n
+        return out
 
-    run(0, max(args.warmup, 1), True)
+    def run(plan_launches, init, events=None):
+        for k, (s0, n) in enumerate(plan_launches):
+            sw.step0, sw.nstep, sw.init_logpsi = s0, n, 1 if (init and k == 0) else 0
+            if events is not None:
+                events[k][0].record()
+            rc = sweep_fn(plan_h, prm_ref, prec, wk_ref, nw, sw_ref, stream)
+            if events is not None:
+                events[k][1].record()
+            if rc:
+                N.check(rc)
+
+    # set-up (untimed): the walkers are thermalised from their atomic start, as
+    # the sampled state a VMC run works on, then the W warm-up steps
+    therm = 200
+    run(launches(0, therm), True)
+    run(launches(therm, max(args.warmup, 1)), False)
     torch.cuda.synchronize()
     nacc.zero_()
-    events = []
+    timed = launches(therm + max(args.warmup, 1), args.steps)
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in timed]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(args.warmup, args.steps, False, events)
+    run(timed, False, events)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = nw * args.steps / elapsed
     acc = float(nacc.sum().item()) / (nw * args.steps)
-    full = [(a.elapsed_time(b) * 1e-3, n) for a, b, n in events if n == spl]
-    launch_s = float(np.mean([t for t, _ in full])) if full else elapsed / max(len(events), 1)
-    steps_per_launch = spl if full else args.steps
+    launch_s = [a.elapsed_time(b) * 1e-3 for a, b in events]
+    device_s = float(np.sum(launch_s))
+    steps_per_launch = timed[0][1]
 
     F, T = algorithmic_cost(mol, wf)
-    flops_launch = (F + 4 * T) * nw * steps_per_launch
-    achieved = flops_launch / launch_s / 1e12
+    flops = (F + 4 * T) * nw * args.steps
+    achieved = flops / device_s / 1e12
+
+    # the table-driven kernels (any basis layout) on the same workload and launch length
+    table = None
+    if plan.specialization is not None:
+        plan.set_specialized(False)
+        try:
+            tl = launches(0, max(args.steps, spl))
+            tev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in tl]
+            run(launches(0, steps_per_launch), False)  # first launch of this instance
+            torch.cuda.synchronize()
+            run(tl, False, tev)
+            torch.cuda.synchronize()
+            tdev = float(np.sum([a.elapsed_time(b) * 1e-3 for a, b in tev]))
+            nst = sum(n for _, n in tl)
+            table = {"walker_steps_per_s_device": nw * nst / tdev, "launch_ms": tdev / len(tl) * 1e3,
+                     "steps_per_launch": tl[0][1], "frac": (F + 4 * T) * nw * nst / tdev / 1e12 / PEAK_FP32_TFLOPS,
+                     "note": "table-driven AO rows (any basis layout); the headline uses the rows generated "
+                             "for this layout (%s)" % plan.specialization}
+        finally:
+            plan.set_specialized(True)
 
     # local-energy throughput on the sampled positions: the FP64 kernel the
     # wavefunction uses for E_L, and the FP32-bulk kernel with its error
@@ -533,16 +588,7 @@ def main():
         torch.cuda.synchronize()
         return reps * nw / (time.perf_counter() - t1)
 
-    el_rate = el_rate_of(wf.eloc_precision)
-    el_rate_fp32 = el_rate_of("mixed")
-    e64 = wf.local_energy_parts(pos, "fp64")["eloc"]
-    emx = wf.local_energy_parts(pos, "mixed")["eloc"]
-    d = ((emx - e64).abs() / mol.nelec)
-    err32 = {"max": float(d.max()), "median": float(d.median()), "mean": float(d.mean()),
-             "bias_of_mean": float((emx.mean() - e64.mean()).abs() / mol.nelec)}
-    e_used = wf.local_energy_parts(pos)["eloc"]
-    du = (e_used - e64).abs() / mol.nelec
-    err = {"max": float(du.max()), "mean": float(du.mean()), "eloc_precision": wf.eloc_precision}
+    el_rate = el_rate_of("fp64")
     sweep_vs64 = sweep_precision_effect(mol, wf, nw) if args.precision == "mixed" else None
 
     traffic, traffic_src = measured_traffic(args, nw, steps_per_launch)
@@ -554,6 +600,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3,
+        "device_ms_per_step": device_s / args.steps * 1e3,
+        "walker_steps_per_s_device": nw * args.steps / device_s,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -568,21 +616,22 @@ def main():
         },
         "acceptance": acc,
         "local_energy_per_s": el_rate,
-        "eloc_error_per_electron_vs_fp64": err,
-        "local_energy_fp32_bulk": {"per_s": el_rate_fp32, "error_per_electron_vs_fp64": err32},
+        "local_energy_precision": "fp64",
         "sweep_fp32_bulk_vs_fp64_chain": sweep_vs64,
         "roofline": {
-            "bound": "mfma",
+            "bound": "valu_fp32",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "sweep_kernel",
-            "launch_ms": launch_s * 1e3,
+            "kernel": "sweep_kernel_wlds" if plan.specialization else "sweep_kernel",
+            "launch_ms": float(np.mean(launch_s)) * 1e3,
+            "launches": len(launch_s),
             "flops_per_walker_step": F + 4 * T,
         },
+        "table_driven": table,
     }
     if not args.no_scan:
         res["walker_scan"] = walker_scan(mol, wf, args.precision)

@@ -26,7 +26,10 @@ enum { QMC_OK = 0, QMC_ERR_ARG = 1, QMC_ERR_HIP = 2, QMC_ERR_UNSUPPORTED = 3 };
 
 enum { QMC_RADIAL_GTO_PURE = 0, QMC_RADIAL_STO = 1 };
 
-enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 }; /* MIXED: FP32 AO/MO/Jastrow, FP64 Slater/energies */
+/* MIXED: psi, pdf, sweeps and drift with FP32 AO/MO/Jastrow and FP64 Slater
+ * elimination, acceptance and walker state.  Local energies (QMC_EVAL_ELOC) are
+ * evaluated in FP64 whatever the precision argument says. */
+enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 };
 
 /* Maximum number of primitives sharing one exponent group, and of unique
  * configurations per spin handled by the kernels. */
@@ -126,6 +129,8 @@ typedef struct QmcWfOut {
     double* kinetic; /* kinetic_energy_jacobi        slater_jastrow.py:312-344 */
     double* eloc;    /* local_energy                 wf_base.py:184-215 */
     double* epot;    /* V_en + V_ee + V_nn           wf_base.py:49-116 */
+    double* v_ee;    /* sum_{i<j} 1/r_ij             WaveFunction.electronic_potential, wf_base.py:49-70 */
+    double* v_en;    /* -sum_{i,A} Z_A/r_iA          WaveFunction.nuclear_potential, wf_base.py:72-95 */
 } QmcWfOut;
 
 enum { QMC_EVAL_PSI = 0, QMC_EVAL_ELOC = 1 };
@@ -276,6 +281,41 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
                        double* djas, double* drift, qmc_stream_t stream);
 
 /*
+ * Parameter gradients of a weighted sum over walkers with the reduction fused
+ * into the adjoint kernels (FP64; mo_w must be float64): for per-walker
+ * weights g,
+ *   qmc_logpsi_grad  L = sum_w g_w log|psi(x_w)|    (g_w = dL/dpsi_w * psi_w)
+ *   qmc_eloc_grad    L = sum_w g_w E_L(x_w)         (eloc [nw] written if non-NULL)
+ * and the gradients dL/dW [nao][nmo] (W = mo_scf * mo_modifier, first nmo
+ * columns), dL/dc [nci], dL/d(w_ee, w_en) [2] and, for log|psi| only,
+ * dL/dbas_exp [nprim_ref] (grad_exp, or NULL to skip).  AO values are
+ * recomputed in the kernel; every wavefront adds its walkers' contributions to
+ * a partial-sum slot of the workspace and one more launch adds the slots, so
+ * no per-walker row reaches memory.  The workspace holds up to 4096 slots of
+ * (nentry nmo + nci + 2 [+ 12 nentry]) doubles; qmc_adjoint_workspace_size
+ * gives the size that uses the most slots, any size of at least one slot
+ * works (fewer slots, each serving more walkers).  The slot count fixes the
+ * order of the sums: equal workspace sizes give bit-identical results.
+ * Replace autograd through SlaterJastrow.forward in Solver.evaluate_grad_manual
+ * (solver.py:372-431) and through WaveFunction.local_energy in
+ * Solver.evaluate_grad_auto (solver.py:352-370, loss.py:49-79).
+ */
+typedef struct QmcGradOut {
+    double* grad_w;   /* [nao * nmo] */
+    double* grad_ci;  /* [nci] */
+    double* grad_jas; /* [2] */
+    double* grad_exp; /* [nprim_ref] or NULL */
+} QmcGradOut;
+
+size_t qmc_adjoint_workspace_size(const QmcPlan* plan, int64_t nw, int32_t with_exp);
+int qmc_logpsi_grad(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                    const double* weights, QmcGradOut* out, void* workspace, size_t workspace_bytes,
+                    qmc_stream_t stream);
+int qmc_eloc_grad(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                  const double* weights, double* eloc, QmcGradOut* out, void* workspace, size_t workspace_bytes,
+                  qmc_stream_t stream);
+
+/*
  * Batched launches: many systems (plans), each with its own parameters,
  * walkers, walker count and step count, in one launch per register bucket
  * (occupied orbitals per spin <= 2, 4, 8, 16, 24) and radial type.  Waves of
@@ -324,6 +364,128 @@ int qmc_wf_eval_batch(QmcBatch* batch, int32_t precision, int32_t mode, const Qm
  * reference (whose proposals come from torch's host generator); exposed so the
  * device generator can be checked against the published known-answer vectors. */
 int qmc_philox4x32_10(const uint32_t* in, uint32_t* out, int64_t n, qmc_stream_t stream);
+
+/*
+ * Parameter views gathered on the device in one launch (the layouts of
+ * QmcParams): W = mo_scf * mo_modifier over the plan's first nmo columns
+ * (molecular_orbitals.py:65-95) in FP64 and FP32, its rows in AO-entry order
+ * padded to the register bucket, and bas_exp gathered in group order
+ * (atomic_orbitals.py:53-55).  mo_scf and mo_modifier are [nao][ld] float64;
+ * any output pointer may be NULL.  Sizes from qmc_plan_sizes.  Replaces the
+ * per-call torch ops that rebuilt these views (no reference counterpart: the
+ * reference recomputes W inside MolecularOrbitals.forward).
+ */
+typedef struct QmcParamViews {
+    double* mo_w64;    /* [nao * nmo] */
+    float* mo_w32;     /* [nao * nmo] */
+    double* entry_w64; /* [nentry * bucket] */
+    float* entry_w32;  /* [nentry * bucket] */
+    double* grp_exp;   /* [nexp_grp] */
+    float* grp_expf;   /* [nexp_grp] */
+} QmcParamViews;
+
+typedef struct QmcPlanSizes {
+    int32_t nexp_grp; /* exponents in group order (length of grp_exp) */
+    int32_t nentry;
+    int32_t bucket;   /* qmc_plan_bucket */
+    int32_t nao;
+    int32_t nmo;
+} QmcPlanSizes;
+
+int qmc_plan_sizes(const QmcPlan* plan, QmcPlanSizes* out);
+int qmc_params_gather(const QmcPlan* plan, const double* mo_scf, const double* mo_modifier, int32_t ld,
+                      const double* bas_exp, QmcParamViews* views, qmc_stream_t stream);
+
+/*
+ * Pade Jastrow factors and potentials at nw walkers (float64; any output may be
+ * NULL).  terms selects the factors: bit 0 the e-e factor, bit 1 the e-n factor
+ * (their product when both; the plan must hold the e-n centres for bit 1):
+ *   value [nw]            J                              (Nbatch, 1) of forward(pos)
+ *   grad  [nw][3][nelec]  dJ/dx_{e,d}                    forward(pos, 1, sum_grad=False)
+ *   lap   [nw][nelec]     sum_d d^2 J / dx_{e,d}^2       forward(pos, 2)
+ *   v_ee  [nw]            sum_{i<j} 1/r_ij               wf_base.py:49-70
+ *   v_en  [nw]            -sum_{i,A} Z_A / r_iA          wf_base.py:72-95
+ * Replaces JastrowFactorElectronElectron.forward
+ * (jastrow_factor_electron_electron.py:124-260), JastrowFactorElectronNuclei.forward
+ * (jastrow_factor_electron_nuclei.py:56-161), CombineJastrow.forward
+ * (combine_jastrow.py:33-195) and WaveFunction.electronic_potential /
+ * nuclear_potential (wf_base.py:49-95).
+ */
+typedef struct QmcJastrowOut {
+    double* value;
+    double* grad;
+    double* lap;
+    double* v_ee;
+    double* v_en;
+} QmcJastrowOut;
+
+int qmc_jastrow_eval(const QmcPlan* plan, const QmcParams* params, int32_t terms, const double* pos, int64_t nw,
+                     QmcJastrowOut* out, qmc_stream_t stream);
+
+/*
+ * Slater determinants and operator traces from caller-given MO matrices
+ * mo [nw][nelec][ld] (float64), for every CI term n of the plan's
+ * configurations: det [nw][nci] = det A_up det A_dn; op_up / op_dn [nop][nw][nci]
+ * = tr(A_s^-1 B_s) (tr((A_s^-1 B_s)^2) with op_squared), A_s and B_s the
+ * configuration's columns of mo and bop [nop][nw][nelec][ld] on the rows of
+ * spin s.  Any output may be NULL.  Replaces SlaterPooling.forward
+ * (slater_pooling.py:64-111) and SlaterPooling.operator (slater_pooling.py:262-409).
+ */
+int qmc_slater_pool(const QmcPlan* plan, const double* mo, int32_t ld, const double* bop, int32_t nop,
+                    int32_t op_squared, int64_t nw, double* det, double* op_up, double* op_dn,
+                    qmc_stream_t stream);
+
+/*
+ * Atomic orbitals with all second derivatives at nw walkers of nelec electrons
+ * (pos [nw][nelec][3]; nelec = 1 for one-electron updates), reference layouts,
+ * float64, any output NULL:
+ *   ao [nw][nelec][nao], grad [..][nao][3], lap [..][nao],
+ *   hess_diag [..][nao][3] (d2/dx2, d2/dy2, d2/dz2), hess_mixed [..][nao][3] (xy, xz, yz).
+ * Replaces AtomicOrbitals.forward with derivative 0 / 1 / 2 / 3 / [0,1,2],
+ * sum_grad, sum_hess and one_elec (atomic_orbitals.py:131-218, 330-575).
+ */
+typedef struct QmcAoOut {
+    double* ao;
+    double* grad;
+    double* lap;
+    double* hess_diag;
+    double* hess_mixed;
+} QmcAoOut;
+
+int qmc_ao_eval_full(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw, int32_t nelec,
+                     QmcAoOut* out, qmc_stream_t stream);
+
+/*
+ * Initial walker positions pos [nw][nelec][3] drawn on the device
+ * (Philox4x32-10 keyed by seed, counted by walker walker0 + w, electron and draw),
+ * for the domains of Molecule.domain (molecule.py:176-214) that
+ * Walkers.initialize accepts (sampler/walkers.py:8-150):
+ *   CENTER   uniform in [lo, hi) per coordinate (lo = -hi = -1e-3 by convention)
+ *   UNIFORM  uniform in [lo, hi) per coordinate
+ *   NORMAL   mean + L z per electron, z ~ N(0, I), L the lower Cholesky factor
+ *            of the covariance
+ *   ATOMIC   electron e at atom a_e + s N(0, I): the assignment a_e is the list
+ *            (atom A repeated atom_nelec[A] times) in a uniformly random order,
+ *            and s = atom_width[3A + r], r = 0 for the first electron of its atom
+ *            in electron order, 1 for the second to fifth, 2 from the sixth on.
+ */
+enum { QMC_INIT_CENTER = 0, QMC_INIT_UNIFORM = 1, QMC_INIT_NORMAL = 2, QMC_INIT_ATOMIC = 3 };
+#define QMC_INIT_MAXATOM 32
+
+typedef struct QmcWalkerInit {
+    int32_t mode;
+    int32_t natom;
+    uint64_t seed;
+    int64_t walker0;
+    double lo, hi;
+    double mean[3];
+    double chol[9];
+    double atom_xyz[3 * QMC_INIT_MAXATOM];
+    double atom_width[3 * QMC_INIT_MAXATOM];
+    int32_t atom_nelec[QMC_INIT_MAXATOM];
+} QmcWalkerInit;
+
+int qmc_walkers_init(const QmcWalkerInit* init, int32_t nelec, int64_t nw, double* pos, qmc_stream_t stream);
 
 /* Text of the last error on the calling thread. */
 const char* qmc_last_error(void);

@@ -62,6 +62,7 @@ class QmcWfOut(ctypes.Structure):
     _fields_ = [
         ("psi", ctypes.c_void_p), ("logabs", ctypes.c_void_p), ("sign", ctypes.c_void_p),
         ("kinetic", ctypes.c_void_p), ("eloc", ctypes.c_void_p), ("epot", ctypes.c_void_p),
+        ("v_ee", ctypes.c_void_p), ("v_en", ctypes.c_void_p),
     ]
 
 
@@ -103,6 +104,46 @@ class QmcEvalJob(ctypes.Structure):
     ]
 
 
+class QmcParamViews(ctypes.Structure):
+    _fields_ = [
+        ("mo_w64", ctypes.c_void_p), ("mo_w32", ctypes.c_void_p), ("entry_w64", ctypes.c_void_p),
+        ("entry_w32", ctypes.c_void_p), ("grp_exp", ctypes.c_void_p), ("grp_expf", ctypes.c_void_p),
+    ]
+
+
+class QmcPlanSizes(ctypes.Structure):
+    _fields_ = [("nexp_grp", ctypes.c_int32), ("nentry", ctypes.c_int32), ("bucket", ctypes.c_int32),
+                ("nao", ctypes.c_int32), ("nmo", ctypes.c_int32)]
+
+
+class QmcJastrowOut(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("lap", ctypes.c_void_p),
+                ("v_ee", ctypes.c_void_p), ("v_en", ctypes.c_void_p)]
+
+
+class QmcAoOut(ctypes.Structure):
+    _fields_ = [("ao", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("lap", ctypes.c_void_p),
+                ("hess_diag", ctypes.c_void_p), ("hess_mixed", ctypes.c_void_p)]
+
+
+class QmcGradOut(ctypes.Structure):
+    _fields_ = [("grad_w", ctypes.c_void_p), ("grad_ci", ctypes.c_void_p), ("grad_jas", ctypes.c_void_p),
+                ("grad_exp", ctypes.c_void_p)]
+
+
+QMC_INIT_CENTER, QMC_INIT_UNIFORM, QMC_INIT_NORMAL, QMC_INIT_ATOMIC = 0, 1, 2, 3
+QMC_INIT_MAXATOM = 32
+
+
+class QmcWalkerInit(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int32), ("natom", ctypes.c_int32), ("seed", ctypes.c_uint64), ("walker0", ctypes.c_int64),
+        ("lo", ctypes.c_double), ("hi", ctypes.c_double), ("mean", ctypes.c_double * 3),
+        ("chol", ctypes.c_double * 9), ("atom_xyz", ctypes.c_double * (3 * QMC_INIT_MAXATOM)),
+        ("atom_width", ctypes.c_double * (3 * QMC_INIT_MAXATOM)), ("atom_nelec", ctypes.c_int32 * QMC_INIT_MAXATOM),
+    ]
+
+
 EXPORTS = {
     "qmc_plan_create": ([ctypes.POINTER(QmcSystemDesc), ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
     "qmc_plan_destroy": ([ctypes.c_void_p], ctypes.c_int),
@@ -132,6 +173,24 @@ EXPORTS = {
     "qmc_wf_eval_batch": ([ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(QmcEvalJob),
                            ctypes.c_int32, ctypes.c_void_p], ctypes.c_int),
     "qmc_philox4x32_10": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
+    "qmc_plan_sizes": ([ctypes.c_void_p, ctypes.POINTER(QmcPlanSizes)], ctypes.c_int),
+    "qmc_params_gather": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                           ctypes.POINTER(QmcParamViews), ctypes.c_void_p], ctypes.c_int),
+    "qmc_jastrow_eval": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_int32, ctypes.c_void_p,
+                          ctypes.c_int64, ctypes.POINTER(QmcJastrowOut), ctypes.c_void_p], ctypes.c_int),
+    "qmc_slater_pool": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                         ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                         ctypes.c_void_p], ctypes.c_int),
+    "qmc_ao_eval_full": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64,
+                          ctypes.c_int32, ctypes.POINTER(QmcAoOut), ctypes.c_void_p], ctypes.c_int),
+    "qmc_adjoint_workspace_size": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32], ctypes.c_size_t),
+    "qmc_logpsi_grad": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                         ctypes.POINTER(QmcGradOut), ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p], ctypes.c_int),
+    "qmc_eloc_grad": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                       ctypes.c_void_p, ctypes.POINTER(QmcGradOut), ctypes.c_void_p, ctypes.c_size_t,
+                       ctypes.c_void_p], ctypes.c_int),
+    "qmc_walkers_init": ([ctypes.POINTER(QmcWalkerInit), ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
+                          ctypes.c_void_p], ctypes.c_int),
     "qmc_last_error": ([], ctypes.c_char_p),
     "qmc_version": ([], ctypes.c_char_p),
 }

@@ -9,6 +9,7 @@ travels with the source tree).  Packed FP32 instructions are disabled
 """
 import argparse
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -24,7 +25,7 @@ _FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep
 ]
 # HMC / drift kernels: single-configuration (MC=1) and multi-configuration (MC=8) instances
 _HMC = [("qmc_hmc_f32.hip", mc) for mc in (1, 8)] + [("qmc_hmc_f64.hip", mc) for mc in (1, 8)]
-UNITS = [("qmc_abi.hip", "qmc_abi", [])] + [
+UNITS = [("qmc_abi.hip", "qmc_abi", []), ("qmc_api.hip", "qmc_api", [])] + [
     (f, f.replace(".hip", "_r%d" % r), ["-DQMC_RAD=%d" % r]) for f in _FAMILIES for r in (0, 1)
 ] + [(f, f.replace(".hip", "_mc%d_r%d" % (mc, r)), ["-DQMC_RAD=%d" % r, "-DQMC_HMC_MC=%d" % mc])
      for f, mc in _HMC for r in (0, 1)]
@@ -66,23 +67,41 @@ def spec_units():
     return [(f, "spec_" + os.path.basename(f)[:-4], []) for f in files]
 
 
-def _stale(unit, deps_mtime: float) -> bool:
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps(path: str, seen=None) -> set:
+    """The file and every quoted include it reaches in csrc/ and include/."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.isfile(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for name in _INCLUDE.findall(f.read()):
+            for d in (os.path.dirname(path), HERE, os.path.join(ROOT, "include")):
+                cand = os.path.join(d, name)
+                if os.path.isfile(cand):
+                    _deps(cand, seen)
+                    break
+    return seen
+
+
+def _stale(unit) -> bool:
     obj = os.path.join(_obj_dir(), unit[1] + ".o")
     src = unit[0] if os.path.isabs(unit[0]) else os.path.join(HERE, unit[0])
-    return not os.path.isfile(obj) or os.path.getmtime(obj) < max(deps_mtime, os.path.getmtime(src))
+    if not os.path.isfile(obj):
+        return True
+    return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in _deps(src))
 
 
 def build(verbose: bool = False, resource_usage: bool = False, jobs: int = 0, incremental: bool = False) -> str:
     """Compile every unit and link the library.  With incremental=True only the
-    units older than their source or any header of csrc/ and include/ are
-    recompiled (a developer shortcut; the default always compiles everything)."""
+    units older than their source or a header they include are recompiled (a
+    developer shortcut; the default always compiles everything)."""
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
     units = UNITS + spec_units()
     if incremental:
-        hdrs = [os.path.join(HERE, f) for f in os.listdir(HERE) if f.endswith(".h")]
-        hdrs += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
-        deps = max(os.path.getmtime(h) for h in hdrs)
-        units = [u for u in units if _stale(u, deps)]
+        units = [u for u in units if _stale(u)]
     # the largest units first, so the parallel build ends sooner
     units = sorted(units, key=lambda u: 0 if u[0].startswith(os.path.join(_obj_dir(), "spec")) else 1)
     jobs = jobs or max(1, min(len(units), os.cpu_count() or 4, 16))

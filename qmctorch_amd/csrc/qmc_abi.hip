@@ -9,17 +9,22 @@
 #include <string>
 #include <vector>
 
-#include "qmc_launch.h"
+#include "qmc_adjoint.h"
+#include "qmc_plan.h"
 
 using namespace qmc;
 
 namespace {
 thread_local std::string g_last_error;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int qmc::set_error(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
+
+namespace {
+int fail(int code, const std::string& msg) { return qmc::set_error(code, msg); }
 
 #define QMC_HIP(call)                                                                                    \
     do {                                                                                                 \
@@ -41,26 +46,6 @@ __global__ void philox_kernel(const uint32_t* in, uint32_t* out, int64_t n) {
 }
 }  // namespace
 
-// =========================================================================== plan
-struct QmcPlan {
-    Tables t;
-    const SpecEntry* spec = nullptr;  // kernels generated for this basis layout, if any
-    bool use_spec = true;
-    std::vector<void*> allocs;
-    ~QmcPlan() {
-        for (void* p : allocs) (void)hipFree(p);
-    }
-    template <typename X>
-    const X* upload(const X* host, size_t n) {
-        if (n == 0 || host == nullptr) return nullptr;
-        void* d = nullptr;
-        if (hipMalloc(&d, n * sizeof(X)) != hipSuccess) throw std::runtime_error("hipMalloc failed");
-        allocs.push_back(d);
-        if (hipMemcpy(d, host, n * sizeof(X), hipMemcpyHostToDevice) != hipSuccess)
-            throw std::runtime_error("hipMemcpy failed");
-        return (const X*)d;
-    }
-};
 
 namespace {
 
@@ -123,7 +108,10 @@ const char* invalid_tables(const QmcSystemDesc* d) {
         !d->conf_up || !d->ci_up || !d->ci_dn || (d->ndown > 0 && !d->conf_dn))
         return "missing table pointer";
     std::vector<int> entry_group(d->nentry, -1);
+    int next_entry = 0;  // the groups' entries follow each other in group order
     for (int g = 0; g < d->ngroup; ++g) {
+        if (d->grp_ao0[g] != next_entry) return "AO entries of the groups must follow in group order";
+        next_entry += std::max(d->grp_nao[g], 0);
         if (d->grp_atom[g] < 0 || d->grp_atom[g] >= d->natom) return "grp_atom out of range";
         const int np = d->grp_nprim[g];
         if (np < 1 || np > QMC_MAXP) return "grp_nprim out of range";
@@ -308,6 +296,7 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.grp_nprim = (decltype(t.grp_nprim))p->upload(d->grp_nprim, d->ngroup);
         t.grp_prim0 = (decltype(t.grp_prim0))p->upload(d->grp_prim0, d->ngroup);
         t.grp_prim_src = (decltype(t.grp_prim_src))p->upload(d->grp_prim_src, nprim_tot);
+        p->nexp_grp = nprim_tot;
         t.grp_ao0 = (decltype(t.grp_ao0))p->upload(d->grp_ao0, d->ngroup);
         t.grp_nao = (decltype(t.grp_nao))p->upload(d->grp_nao, d->ngroup);
         t.grp_tight = (decltype(t.grp_tight))p->upload(d->grp_tight, d->grp_tight ? d->ngroup : 0);
@@ -385,6 +374,7 @@ int qmc_wf_eval(const QmcPlan* plan, const QmcParams* params, int32_t precision,
     if (!pos || !out) return fail(QMC_ERR_ARG, "null pos/out");
     if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
     if (nw == 0) return QMC_OK;
+    if (mode == QMC_EVAL_ELOC) precision = QMC_PREC_FP64;  // local energies are FP64 in every precision
     KArgs a;
     a.t = plan->t;
     a.p = *params;
@@ -529,8 +519,8 @@ int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const dou
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
     if ((rc = refuse_excitations(plan))) return rc;
+    if (nw == 0) return QMC_OK;  // an empty batch: its tensors may have no storage
     if (!pos || !ao_rows || !gamma_rows || !dci || !djas) return fail(QMC_ERR_ARG, "null output");
-    if (nw == 0) return QMC_OK;
     GArgs a;
     a.t = plan->t;
     a.p = *params;
@@ -549,6 +539,78 @@ int qmc_logpsi_grad_rows(const QmcPlan* plan, const QmcParams* params, const dou
     return QMC_OK;
 }
 
+size_t qmc_adjoint_workspace_size(const QmcPlan* plan, int64_t nw, int32_t with_exp) {
+    if (!plan || nw < 0) return 0;
+    const RedSlots r = red_layout(plan->t, nw, with_exp != 0, nullptr, (size_t)-1, nullptr);
+    return (size_t)(r.nslot > 0 ? r.nslot : 1) * r.stride * sizeof(double);
+}
+
+int qmc_logpsi_grad(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                    const double* weights, QmcGradOut* out, void* workspace, size_t workspace_bytes,
+                    qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
+    if (!out || !out->grad_w || !out->grad_ci || !out->grad_jas) return fail(QMC_ERR_ARG, "null gradient output");
+    const RedSlots r = red_layout(plan->t, nw, out->grad_exp != nullptr, (double*)workspace, workspace_bytes,
+                                  weights);
+    if (nw > 0 && (!pos || !weights)) return fail(QMC_ERR_ARG, "null pos or weights");
+    if (nw > 0 && (!workspace || r.nslot < 1))
+        return fail(QMC_ERR_ARG, "workspace smaller than one partial-sum slot (qmc_adjoint_workspace_size)");
+    hipStream_t s = (hipStream_t)stream;
+    RedOut o{out->grad_w, out->grad_ci, out->grad_jas, out->grad_exp};
+    if (nw == 0) {  // empty batch: zero gradients
+        fill_zero(o.grad_w, (size_t)plan->t.nao * plan->t.nmo, s);
+        fill_zero(o.grad_ci, plan->t.nci, s);
+        fill_zero(o.grad_jas, 2, s);
+        if (o.grad_exp) fill_zero(o.grad_exp, plan->t.nprim_ref, s);
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
+    GRedArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.r = r;
+    if (QMC_BY_RAD(launch_grad_reduce, a.t.radial, a, o, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
+int qmc_eloc_grad(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                  const double* weights, double* eloc, QmcGradOut* out, void* workspace, size_t workspace_bytes,
+                  qmc_stream_t stream) {
+    int rc = check_common(plan, params, nw);
+    if (rc) return rc;
+    if ((rc = refuse_excitations(plan))) return rc;
+    if (!out || !out->grad_w || !out->grad_ci || !out->grad_jas) return fail(QMC_ERR_ARG, "null gradient output");
+    if (out->grad_exp) return fail(QMC_ERR_UNSUPPORTED, "d E_L / d bas_exp is not provided");
+    const RedSlots r = red_layout(plan->t, nw, false, (double*)workspace, workspace_bytes, weights);
+    if (nw > 0 && (!pos || !weights)) return fail(QMC_ERR_ARG, "null pos or weights");
+    if (nw > 0 && (!workspace || r.nslot < 1))
+        return fail(QMC_ERR_ARG, "workspace smaller than one partial-sum slot (qmc_adjoint_workspace_size)");
+    hipStream_t s = (hipStream_t)stream;
+    RedOut o{out->grad_w, out->grad_ci, out->grad_jas, nullptr};
+    if (nw == 0) {
+        fill_zero(o.grad_w, (size_t)plan->t.nao * plan->t.nmo, s);
+        fill_zero(o.grad_ci, plan->t.nci, s);
+        fill_zero(o.grad_jas, 2, s);
+        QMC_HIP(hipGetLastError());
+        return QMC_OK;
+    }
+    ERedArgs a;
+    a.t = plan->t;
+    a.p = *params;
+    a.pos = pos;
+    a.nw = nw;
+    a.eloc = eloc;
+    a.r = r;
+    if (QMC_BY_RAD(launch_eloc_reduce, a.t.radial, a, o, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    QMC_HIP(hipGetLastError());
+    return QMC_OK;
+}
+
 int qmc_philox4x32_10(const uint32_t* in, uint32_t* out, int64_t n, qmc_stream_t stream) {
     if (!in || !out || n < 0) return fail(QMC_ERR_ARG, "bad philox arguments");
     if (n == 0) return QMC_OK;
@@ -563,9 +625,9 @@ int qmc_eloc_grad_rows(const QmcPlan* plan, const QmcParams* params, const doubl
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
     if ((rc = refuse_excitations(plan))) return rc;
-    if (!pos) return fail(QMC_ERR_ARG, "null pos");
     if ((ao_rows == nullptr) != (b_rows == nullptr)) return fail(QMC_ERR_ARG, "ao_rows and b_rows go together");
-    if (nw == 0) return QMC_OK;
+    if (nw == 0) return QMC_OK;  // an empty batch: its tensors may have no storage
+    if (!pos) return fail(QMC_ERR_ARG, "null pos");
     EArgs a;
     a.t = plan->t;
     a.p = *params;
@@ -759,6 +821,7 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
                       qmc_stream_t stream) {
     if (!b || (njobs > 0 && !jobs)) return fail(QMC_ERR_ARG, "null batch or jobs");
     if (njobs < 0 || njobs > b->max_jobs) return fail(QMC_ERR_ARG, "job count outside [0, max_jobs]");
+    if (mode == QMC_EVAL_ELOC) precision = QMC_PREC_FP64;  // local energies are FP64 in every precision
     std::vector<QmcEvalJob> live;
     for (int j = 0; j < njobs; ++j) {
         const QmcEvalJob& x = jobs[j];

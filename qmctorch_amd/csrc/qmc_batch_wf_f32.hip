@@ -1,4 +1,4 @@
-// Batched psi / local-energy evaluation (many systems per launch), float bulk precision.
+// Batched psi evaluation (many systems per launch), float bulk precision (local energies: FP64 units).
 #include "qmc_launch.h"
 
 #ifndef QMC_RAD
@@ -9,12 +9,10 @@ namespace qmc {
 
 template <int NM, int RAD>
 static void go(const WJob* jobs, int njobs, int nwaves, int mode, hipStream_t s) {
-    if (mode == QMC_EVAL_ELOC)
-        hipLaunchKernelGGL((wf_batch_kernel<float, NM, RAD, true>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs,
-                           njobs, nwaves);
-    else
-        hipLaunchKernelGGL((wf_batch_kernel<float, NM, RAD, false>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs,
-                           njobs, nwaves);
+    // psi only: local energies are evaluated by the FP64 kernels in every precision
+    (void)mode;
+    hipLaunchKernelGGL((wf_batch_kernel<float, NM, RAD, false>), grid_for_waves(nwaves), dim3(kBlock), 0, s, jobs,
+                       njobs, nwaves);
 }
 
 template <>

@@ -1,4 +1,5 @@
 // Adjoint-row and AO-layout kernels (FP64).
+#include "qmc_adjoint.h"
 #include "qmc_launch.h"
 
 #ifndef QMC_RAD
@@ -59,6 +60,41 @@ static int bucket_e(const EArgs& a, hipStream_t s) {
 template <>
 int launch_eloc_grad<QMC_RAD>(const EArgs& a, hipStream_t s) {
     return bucket_e<QMC_RAD>(a, s);
+}
+
+static void slot_sum(const Tables& t, const RedSlots& r, const RedOut& o, hipStream_t s) {
+    const int n = t.nao * t.nmo + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
+    hipLaunchKernelGGL(slot_reduce_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t, r, o);
+}
+
+template <>
+int launch_grad_reduce<QMC_RAD>(const GRedArgs& a, const RedOut& o, hipStream_t s) {
+    const dim3 grid = grid_for_waves(a.r.nslot);
+    switch (nm_bucket(a.t)) {
+        case 2: hipLaunchKernelGGL((grad_reduce_kernel<2, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((grad_reduce_kernel<4, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((grad_reduce_kernel<8, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((grad_reduce_kernel<16, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 24: hipLaunchKernelGGL((grad_reduce_kernel<24, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        default: return -1;
+    }
+    slot_sum(a.t, a.r, o, s);
+    return 0;
+}
+
+template <>
+int launch_eloc_reduce<QMC_RAD>(const ERedArgs& a, const RedOut& o, hipStream_t s) {
+    const dim3 grid = grid_for_waves(a.r.nslot);
+    switch (nm_bucket(a.t)) {
+        case 2: hipLaunchKernelGGL((eloc_grad_reduce_kernel<2, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((eloc_grad_reduce_kernel<4, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((eloc_grad_reduce_kernel<8, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((eloc_grad_reduce_kernel<16, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        case 24: hipLaunchKernelGGL((eloc_grad_reduce_kernel<24, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
+        default: return -1;
+    }
+    slot_sum(a.t, a.r, o, s);
+    return 0;
 }
 
 }  // namespace qmc

@@ -127,13 +127,14 @@ struct MoSink {
 // Pade Jastrow terms of one electron (jastrow_factor_electron_electron.py:124-260,
 // jastrow_factor_electron_nuclei.py:56-161, pade kernels): value contributions,
 // gradient of ln J with respect to this electron and its Laplacian of ln J;
-// also the potential terms of wf_base.py:49-95 (V_ee counted half per lane).
+// also the potential terms of wf_base.py:49-95: vee (this lane's e-e pairs,
+// counted half per lane) and ven (this electron's -Z/r terms).
 template <bool DERIV>
 __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const QmcParams& prm, const WalkerLane& L,
                                                       double x, double y, double z, double& kee, double& ken,
                                                       double& gx, double& gy, double& gz, double& lap,
-                                                      double& vpot) {
-    kee = ken = gx = gy = gz = lap = vpot = 0.0;
+                                                      double& vee, double& ven) {
+    kee = ken = gx = gy = gz = lap = vee = ven = 0.0;
     const int Ne = t.nelec;
     const double wee = (t.use_ee && prm.w_ee) ? ldc(prm.w_ee, 0) : 0.0;
     const double wen = (t.use_en && prm.w_en) ? ldc(prm.w_en, 0) : 0.0;
@@ -155,7 +156,7 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
                 const double dx = x - xj, dy = y - yj, dz = z - zj;
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 const double ir = 1.0 / r;
-                if (DERIV) vpot += 0.5 * ir;
+                if (DERIV) vee += 0.5 * ir;
                 if (t.use_ee) {
                     const bool same = (j < t.nup) == (L.e < t.nup);
                     const double w0 = same ? 0.25 : 0.5;
@@ -194,7 +195,7 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
         for (int A = 0; A < t.natom; ++A) {
             const double dx = x - ldc(prm.atom_coords, 3 * A), dy = y - ldc(prm.atom_coords, 3 * A + 1),
                          dz = z - ldc(prm.atom_coords, 3 * A + 2);
-            vpot -= t.atom_z[A] / sqrt(dx * dx + dy * dy + dz * dz);
+            ven -= t.atom_z[A] / sqrt(dx * dx + dy * dy + dz * dz);
         }
     }
 }
@@ -570,8 +571,8 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
     MoSink<AT, NM, ELOC, AT> sink{(const AT*)a.p.entry_w, t.nmo, mo, gx, gy, gz, lp};
     AO::template row<T, RAD, ELOC, ELOC>(t, a.p, x, y, z, sink);
 
-    double kee, ken, jgx, jgy, jgz, jlap, vpot;
-    jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vpot);
+    double kee, ken, jgx, jgy, jgz, jlap, vee, ven;
+    jastrow_and_potential<ELOC>(t, a.p, L, x, y, z, kee, ken, jgx, jgy, jgz, jlap, vee, ven);
 
     double bk[NM];
     if (ELOC) {
@@ -587,8 +588,12 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
     else
         slater_combine<NM, ELOC>(t, a.p, L, mo, bk, logabs, sgn, kin);
     const double logj = seg_sum(0.5 * kee + ken, L.wbase, t.nelec, t.nelec);
-    double epot = 0.0;
-    if (ELOC) epot = seg_sum(vpot, L.wbase, t.nelec, t.nelec) + t.vnn;
+    double epot = 0.0, svee = 0.0, sven = 0.0;
+    if (ELOC) {
+        svee = seg_sum(vee, L.wbase, t.nelec, t.nelec);
+        sven = seg_sum(ven, L.wbase, t.nelec, t.nelec);
+        epot = svee + sven + t.vnn;
+    }
     if (L.valid && L.e == 0) {
         const double la = logabs + logj;
         if (a.out.logabs) a.out.logabs[L.w] = la;
@@ -598,6 +603,8 @@ __device__ __forceinline__ void wf_walkers(const KArgs& a, const WalkerLane& L) 
             if (a.out.kinetic) a.out.kinetic[L.w] = kin;
             if (a.out.epot) a.out.epot[L.w] = epot;
             if (a.out.eloc) a.out.eloc[L.w] = kin + epot;
+            if (a.out.v_ee) a.out.v_ee[L.w] = svee;
+            if (a.out.v_en) a.out.v_en[L.w] = sven;
         }
     }
 }
@@ -1772,9 +1779,9 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_kernel(EArgs a) {
     const double sdte = seg_sum(dte, L.wbase, t.nelec, t.nelec);
     const double sdtn = seg_sum(dtn, L.wbase, t.nelec, t.nelec);
     if (a.eloc) {
-        double kee, ken, d0, d1, d2, d3, vpot;
-        jastrow_and_potential<true>(t, a.p, L, x, y, z, kee, ken, d0, d1, d2, d3, vpot);
-        const double ep = seg_sum(vpot, L.wbase, t.nelec, t.nelec) + t.vnn;
+        double kee, ken, d0, d1, d2, d3, vee, ven;
+        jastrow_and_potential<true>(t, a.p, L, x, y, z, kee, ken, d0, d1, d2, d3, vee, ven);
+        const double ep = seg_sum(vee + ven, L.wbase, t.nelec, t.nelec) + t.vnn;
         if (v && L.e == 0) a.eloc[L.w] = T + ep;
     }
     if (v && L.e == 0 && a.djas) {

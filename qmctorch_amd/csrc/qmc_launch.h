@@ -47,6 +47,12 @@ template <int RAD> int launch_sweep_f64(const SArgs& a, hipStream_t s);
 template <int RAD> int launch_grad(const GArgs& a, hipStream_t s);
 template <int RAD> int launch_ao(const AArgs& a, bool deriv, hipStream_t s);
 template <int RAD> int launch_eloc_grad(const EArgs& a, hipStream_t s);
+// fused adjoints (qmc_adjoint.h): the reduction kernel plus the slot sum
+struct GRedArgs;
+struct ERedArgs;
+struct RedOut;
+template <int RAD> int launch_grad_reduce(const GRedArgs& a, const RedOut& o, hipStream_t s);
+template <int RAD> int launch_eloc_reduce(const ERedArgs& a, const RedOut& o, hipStream_t s);
 // excitation (trace-trick) kernels, any precision; register buckets up to 16
 template <int RAD> int launch_wf_exc(const KArgs& a, int precision, int mode, hipStream_t s);
 template <int RAD> int launch_sweep_exc(const SArgs& a, int precision, hipStream_t s);

@@ -1,4 +1,4 @@
-// psi / local-energy kernels, float bulk precision.
+// psi kernels, float bulk precision (local energies: qmc_wf_f64.hip).
 #include "qmc_launch.h"
 
 #ifndef QMC_RAD
@@ -9,11 +9,9 @@ namespace qmc {
 
 template <int NM, int RAD>
 static void go(const KArgs& a, int mode, hipStream_t s) {
-    const dim3 g = grid_for(a.t, a.nw);
-    if (mode == QMC_EVAL_ELOC)
-        hipLaunchKernelGGL((wf_kernel<float, NM, RAD, true>), g, dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((wf_kernel<float, NM, RAD, false>), g, dim3(kBlock), 0, s, a);
+    // psi only: local energies are evaluated by the FP64 kernels in every precision
+    (void)mode;
+    hipLaunchKernelGGL((wf_kernel<float, NM, RAD, false>), grid_for(a.t, a.nw), dim3(kBlock), 0, s, a);
 }
 
 template <int RAD>

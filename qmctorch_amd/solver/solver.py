@@ -170,7 +170,9 @@ class Solver(SolverBase):
             raise ValueError("Manual gradient only for energy minimization")
         with torch.no_grad():
             eloc = self.wf.local_energy(lpos)
-        psi = self.wf(lpos)
+        # the sampled positions carry requires_grad (metropolis.py:177); the
+        # parameter gradients do not need d psi / d x, so it is not formed
+        psi = self.wf(lpos.detach())
         norm = 1.0 / len(psi)
         weight = eloc.clone()
         weight -= torch.mean(eloc)

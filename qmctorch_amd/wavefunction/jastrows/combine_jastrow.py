@@ -1,7 +1,8 @@
 """Product of Jastrow factors (jastrows/combine_jastrow.py:7-195).
 
 State-dict keys follow the reference (``jastrow.jastrow_terms.<i>.jastrow_kernel.weight``);
-the product rule is applied inside the fused kernels by summing the log-terms.
+the product rule is applied by summing the log-terms, inside the fused
+wavefunction kernels or, for ``forward``, in ``qmc_jastrow_eval``.
 """
 from typing import List
 
@@ -29,4 +30,15 @@ class CombineJastrow(nn.Module):
         return list(self.jastrow_terms)
 
     def forward(self, pos, derivative=0, sum_grad=True):
-        raise NotImplementedError("combined Jastrow factors are evaluated inside the fused wavefunction kernels")
+        """Product of the factors and its derivatives (combine_jastrow.py:33-106):
+        J (Nbatch, 1); derivative=1 as the single factors; derivative=2 the
+        Laplacian of the product; [0, 1, 2] returns the gradient per axis
+        (Nbatch, 3, Nelec) whatever sum_grad is, as the reference does."""
+        from ._device import evaluate, shaped
+
+        ee = next((t for t in self.jastrow_terms if t.kind == "ee"), None)
+        en = next((t for t in self.jastrow_terms if t.kind == "en"), None)
+        mol = (ee or en).mol
+        need = derivative if isinstance(derivative, list) else [derivative]
+        j, dj, d2j = evaluate(self, mol, ee, en, pos, 1 in need, 2 in need)
+        return shaped(j, dj, d2j, derivative, sum_grad, full_grad_in_tuple=True)

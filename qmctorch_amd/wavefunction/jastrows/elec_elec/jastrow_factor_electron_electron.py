@@ -1,8 +1,9 @@
 """Electron-electron Jastrow factor (jastrow_factor_electron_electron.py:11-260).
 
-Holds the kernel module (state_dict key ``jastrow.jastrow_kernel.weight``) and
-the metadata the fused kernels need; J = exp(sum_{i<j} B(r_ij)) and its
-derivatives are evaluated on the device by the wavefunction kernels.
+Holds the kernel module (state_dict key ``jastrow.jastrow_kernel.weight``);
+inside a wavefunction J = exp(sum_{i<j} B(r_ij)) and its derivatives are
+evaluated by the fused wavefunction kernels, and ``forward`` evaluates them on
+their own (``qmc_jastrow_eval``) with the reference's output shapes.
 """
 import torch
 from torch import nn
@@ -27,12 +28,18 @@ class JastrowFactorElectronElectron(nn.Module):
         self.device = torch.device("cuda" if cuda else "cpu")
         self.jastrow_kernel = jastrow_kernel(mol.nup, mol.ndown, cuda, **kernel_kwargs)
         self.requires_autograd = self.jastrow_kernel.requires_autograd
+        self.mol = mol
 
     def __repr__(self) -> str:
         return "ee -> " + self.jastrow_kernel.__class__.__name__
 
     def forward(self, pos, derivative=0, sum_grad=True):
-        raise NotImplementedError(
-            "the e-e Jastrow is evaluated inside the fused wavefunction kernels "
-            "(SlaterJastrow.forward / pdf / local_energy)"
-        )
+        """J (Nbatch, 1); derivative=1: dJ summed over x, y, z (Nbatch, Nelec) or
+        per axis (Nbatch, 3, Nelec) with sum_grad=False; derivative=2: the
+        Laplacian of J per electron (Nbatch, Nelec); [0, 1, 2]: the three
+        (jastrow_factor_electron_electron.py:124-175)."""
+        from .._device import evaluate, shaped
+
+        need = derivative if isinstance(derivative, list) else [derivative]
+        j, dj, d2j = evaluate(self, self.mol, self, None, pos, 1 in need, 2 in need)
+        return shaped(j, dj, d2j, derivative, sum_grad)

@@ -29,3 +29,17 @@ class PadeJastrowKernel(nn.Module):
     def forward(self, r: torch.Tensor) -> torch.Tensor:
         """Kernel values on a (Nbatch, Npair) tensor of distances."""
         return self.static_weight.to(r) * r / (1.0 + self.weight * r)
+
+    def compute_derivative(self, r: torch.Tensor, dr: torch.Tensor) -> torch.Tensor:
+        """d B / d x_k = w0 (dr/dx_k) / (1 + w r)^2 per axis, shape of dr
+        (Nbatch, 3, ...) (pade_jastrow_kernel.py:84-117): the
+        reference's two terms w0 dr/(1+wr) - w0 w r dr/(1+wr)^2 combined."""
+        q = 1.0 / (1.0 + self.weight * r.unsqueeze(1))
+        return self.static_weight.to(r) * dr * q * q
+
+    def compute_second_derivative(self, r: torch.Tensor, dr: torch.Tensor, d2r: torch.Tensor) -> torch.Tensor:
+        """d^2 B / d x_k^2 = w0 [d2r / (1 + w r)^2 - 2 w dr^2 / (1 + w r)^3]
+        (pade_jastrow_kernel.py:119-153, its four terms combined)."""
+        q = 1.0 / (1.0 + self.weight * r.unsqueeze(1))
+        q2 = q * q
+        return self.static_weight.to(r) * (d2r * q2 - 2.0 * self.weight * dr * dr * q2 * q)

@@ -2,7 +2,10 @@
 
 Same parameters and attributes (``atom_coords``, ``bas_exp`` trainable,
 ``norm_cst`` computed once at construction like atomic_orbitals.py:90-94);
-``forward`` evaluates on the device through ``qmc_ao_eval``.
+``forward`` evaluates on the device through ``qmc_ao_eval_full`` for every
+derivative option of the reference: values, gradients (summed or per axis),
+Laplacians or Hessian diagonals, mixed second derivatives, all three at once,
+and one-electron updates.
 """
 from typing import List, Optional
 
@@ -11,6 +14,7 @@ import torch
 from torch import nn
 
 from ... import _native as N
+from ..param_views import f64
 from ..plan import DevicePlan, SystemTables, atomic_orbital_norm
 
 
@@ -48,56 +52,84 @@ class AtomicOrbitals(nn.Module):
         return "AtomicOrbitals(%s, %s, %d -> (%d,%d) )" % (
             self.radial_type, self.harmonics_type, self.nelec * self.ndim, self.nelec, self.norb)
 
-    # plan shared with the owning wavefunction when there is one
     def device_plan(self, owner_plan=None):
+        """The owning wavefunction's plan, or one of this layer's (rebuilt when
+        the exponents stop sharing their groups)."""
         if owner_plan is not None:
             return owner_plan
         key = self.bas_exp._version
         if self._plan is None or self._plan_key != key:
-            ex = self.bas_exp.detach().cpu().numpy()
-            tab = SystemTables(self.mol.basis, self.mol.atom_coords, self.mol.atomic_number, self.mol.nup,
-                               self.mol.ndown, (np.arange(self.mol.nup)[None], np.arange(self.mol.ndown)[None]),
-                               max(self.mol.nup, 1), False, False, self.norm_cst.cpu().numpy(), ex)
-            self._plan = DevicePlan(tab)
+            ex = self.bas_exp.detach().cpu().numpy().astype(np.float64)
+            if self._plan is None or not self._plan.tables.grouping_valid(ex):
+                tab = SystemTables(self.mol.basis, self.mol.atom_coords, self.mol.atomic_number, self.mol.nup,
+                                   self.mol.ndown, (np.arange(self.mol.nup)[None], np.arange(self.mol.ndown)[None]),
+                                   max(self.mol.nup, 1), False, False, self.norm_cst.cpu().numpy(), ex)
+                self._plan = DevicePlan(tab)
             self._plan_key = key
         return self._plan
 
-    def forward(self, pos: torch.Tensor, derivative: Optional[List[int]] = [0], sum_grad: Optional[bool] = True,
-                sum_hess: Optional[bool] = True, one_elec: Optional[bool] = False, plan=None):
-        """AO values (Nw, Ne, Nao) or derivatives, atomic_orbitals.py:131-218."""
-        if not isinstance(derivative, list):
-            derivative = [derivative]
-        if one_elec:
-            raise NotImplementedError("one_elec AO updates are not provided by the HIP path")
+    def _evaluate(self, pos, nelec, want, plan=None):
         N.require_device(pos, "positions")
         dp = self.device_plan(plan)
-        nw = pos.shape[0]
-        x = pos.detach().to(torch.float64).contiguous()
-        need_d = any(d in derivative for d in (1, 2))
-        ao = torch.empty(nw, self.nelec, self.norb, dtype=torch.float64, device=pos.device)
-        dao = torch.empty(nw, self.nelec, self.norb, 3, dtype=torch.float64, device=pos.device) if need_d else None
-        d2ao = torch.empty(nw, self.nelec, self.norb, dtype=torch.float64, device=pos.device) if need_d else None
-        prm = N.QmcParams()
-        be = self.bas_exp.detach().to(torch.float64).contiguous()
-        ac = self.atom_coords.detach().to(torch.float64).contiguous()
-        src = torch.as_tensor(dp.tables.grp_prim_src.astype(np.int64), device=be.device)
-        gexp = be.index_select(0, src)
-        gexpf = gexp.to(torch.float32)
-        prm.bas_exp, prm.atom_coords = be.data_ptr(), ac.data_ptr()
-        prm.grp_exp, prm.grp_expf = gexp.data_ptr(), gexpf.data_ptr()
-        N.check(N.load().qmc_ao_eval(dp.handle, prm, x.data_ptr(), nw, ao.data_ptr(), N.ptr(dao), N.ptr(d2ao),
-                                     N.current_stream()))
+        x = f64(pos)
+        nw = x.shape[0]
+        dev = x.device
+        shape = (nw, nelec, self.norb)
+        out = {k: torch.empty(*shape, *( (3,) if k in ("grad", "hess_diag", "hess_mixed") else ()),
+                              dtype=torch.float64, device=dev) for k in want}
+        if nw:
+            be = f64(self.bas_exp)
+            ac = f64(self.atom_coords)
+            src = getattr(dp, "_grp_src_dev", None)
+            if src is None or src.device != dev:
+                src = dp._grp_src_dev = torch.as_tensor(dp.tables.grp_prim_src.astype(np.int64), device=dev)
+            gexp = be.index_select(0, src)
+            prm = N.QmcParams()
+            prm.bas_exp, prm.atom_coords, prm.grp_exp = be.data_ptr(), ac.data_ptr(), gexp.data_ptr()
+            o = N.QmcAoOut()
+            for k, v in out.items():
+                setattr(o, k, v.data_ptr())
+            N.check(N.load().qmc_ao_eval_full(dp.handle, prm, x.data_ptr(), nw, nelec, o, N.current_stream()))
+        return out
+
+    def forward(self, pos: torch.Tensor, derivative: Optional[List[int]] = [0], sum_grad: Optional[bool] = True,
+                sum_hess: Optional[bool] = True, one_elec: Optional[bool] = False, plan=None):
+        """AO values (Nbatch, Nelec, Nao) or derivatives (atomic_orbitals.py:131-218):
+        [1] summed gradient (Nbatch, Nelec, Nao) or per axis (..., 3) with
+        sum_grad=False; [2] Laplacian, or the Hessian diagonal (..., 3) with
+        sum_hess=False; [3] mixed second derivatives xy, xz, yz (..., 3);
+        [0, 1, 2] (values, per-axis gradients, Laplacians).  one_elec=True takes
+        the positions of one electron (Nbatch, 3)."""
+        if not isinstance(derivative, list):
+            derivative = [derivative]
+        if not sum_grad:
+            assert 1 in derivative
+        if not sum_hess:
+            assert 2 in derivative
+        nelec = 1 if one_elec else self.nelec
+        if pos.dim() != 2 or pos.shape[1] != 3 * nelec:
+            raise ValueError("positions must have shape (Nbatch, %d)" % (3 * nelec))
         dt = torch.get_default_dtype()
-        out = []
-        for d in derivative:
-            if d == 0:
-                out.append(ao.to(dt))
-            elif d == 1:
-                out.append((dao.sum(-1) if sum_grad else dao).to(dt))
-            elif d == 2:
-                if not sum_hess:
-                    raise NotImplementedError("per-axis Hessian diagonal is not provided by the HIP path")
-                out.append(d2ao.to(dt))
-            else:
-                raise NotImplementedError("derivative %s" % d)
-        return out[0] if len(out) == 1 else tuple(out)
+        if derivative == [0]:
+            return self._evaluate(pos, nelec, ["ao"], plan)["ao"].to(dt)
+        if derivative == [1]:
+            g = self._evaluate(pos, nelec, ["grad"], plan)["grad"]
+            return (g.sum(-1) if sum_grad else g).to(dt)
+        if derivative == [2]:
+            if sum_hess:
+                return self._evaluate(pos, nelec, ["lap"], plan)["lap"].to(dt)
+            return self._evaluate(pos, nelec, ["hess_diag"], plan)["hess_diag"].to(dt)
+        if derivative == [3]:
+            return self._evaluate(pos, nelec, ["hess_mixed"], plan)["hess_mixed"].to(dt)
+        if derivative == [0, 1, 2]:
+            o = self._evaluate(pos, nelec, ["ao", "grad", "lap"], plan)
+            return o["ao"].to(dt), o["grad"].to(dt), o["lap"].to(dt)
+        raise ValueError("derivative must be 0, 1, 2, 3 or [0, 1, 2], got ", derivative)
+
+    def update(self, ao: torch.Tensor, pos: torch.Tensor, idelec: int) -> torch.Tensor:
+        """AO matrix with electron idelec's row recomputed at its new position
+        (atomic_orbitals.py:671-695)."""
+        ao_new = ao.clone()
+        ids, ide = idelec * 3, (idelec + 1) * 3
+        ao_new[:, idelec, :] = self.forward(pos[:, ids:ide], one_elec=True).squeeze(1)
+        return ao_new

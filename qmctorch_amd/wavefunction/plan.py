@@ -79,7 +79,7 @@ class SystemTables:
     TIGHT_EXPONENT = 50.0
 
     def __init__(self, basis, atom_coords, atomic_number, nup, ndown, configs, nmo_used,
-                 use_ee, use_en, norm_cst, bas_exp_values):
+                 use_ee, use_en, norm_cst, bas_exp_values, vnn_coords=None):
         self.basis = basis
         self.nup, self.ndown = int(nup), int(ndown)
         self.nelec = self.nup + self.ndown
@@ -177,12 +177,11 @@ class SystemTables:
 
         coords = np.asarray(atom_coords, dtype=np.float64).reshape(-1, 3)
         self.atom_z = np.asarray(atomic_number, dtype=np.float64)
+        # e-n Jastrow centres: the molecule's coordinates at construction
+        # (jastrow_factor_electron_nuclei.py:40); V_nn from the AO layer's
+        # current coordinates when given (wf_base.py:97-116)
         self.jastrow_atoms = np.ascontiguousarray(coords).astype(np.float64)
-        vnn = 0.0
-        for a in range(self.natom):
-            for b in range(a + 1, self.natom):
-                vnn += self.atom_z[a] * self.atom_z[b] / np.linalg.norm(coords[a] - coords[b])
-        self.vnn = float(vnn)
+        self.vnn = nuclear_repulsion(coords if vnn_coords is None else vnn_coords, self.atom_z)
 
         cup, cdn = [np.asarray(c, dtype=np.int64) for c in configs]
         self.nci = cup.shape[0]
@@ -241,6 +240,33 @@ class SystemTables:
         return d
 
 
+def nuclear_repulsion(coords, charges) -> float:
+    """sum_{A<B} Z_A Z_B / |R_A - R_B| (wf_base.py:97-116)."""
+    c = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+    z = np.asarray(charges, dtype=np.float64)
+    vnn = 0.0
+    for a in range(len(z)):
+        for b in range(a + 1, len(z)):
+            vnn += z[a] * z[b] / np.linalg.norm(c[a] - c[b])
+    return float(vnn)
+
+
+def component_plan(mol, configs=None, nmo_used=None, use_ee=False, use_en=False) -> "DevicePlan":
+    """Plan of a module used on its own (atomic orbitals, Jastrow factors,
+    Slater pooling): the molecule's basis tables, the given configurations
+    (ground state by default) and Jastrow flags."""
+    if configs is None:
+        configs = (np.arange(mol.nup)[None], np.arange(mol.ndown)[None])
+    cup, cdn = [np.asarray(c, dtype=np.int64) for c in configs]
+    if nmo_used is None:
+        nmo_used = max(int(cup.max()) + 1 if cup.size else 1, int(cdn.max()) + 1 if cdn.size else 1,
+                       mol.nup, max(mol.ndown, 1))
+    b = mol.basis
+    tab = SystemTables(b, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown, (cup, cdn), nmo_used, use_ee,
+                       use_en, atomic_orbital_norm(b), np.asarray(b.bas_exp, dtype=np.float64))
+    return DevicePlan(tab)
+
+
 def _unique_rows(c: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
     """Sorted unique rows and inverse index (torch.unique(dim=0) semantics,
     orbital_projector.py:29-54)."""
@@ -270,6 +296,13 @@ class DevicePlan:
                 self.handle = ctypes.c_void_p()
         except Exception:
             pass
+
+    def sizes(self) -> N.QmcPlanSizes:
+        if getattr(self, "_sizes", None) is None:
+            sz = N.QmcPlanSizes()
+            N.check(self._lib.qmc_plan_sizes(self.handle, ctypes.byref(sz)))
+            self._sizes = sz
+        return self._sizes
 
     @property
     def bucket(self) -> int:

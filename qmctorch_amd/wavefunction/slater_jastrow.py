@@ -9,8 +9,9 @@ libqmcmi355x (``qmc_wf_eval``); parameter gradients of ``forward`` come from
 the adjoint kernel ``qmc_logpsi_grad_rows`` instead of autograd.
 
 Extra keyword ``precision``: ``"fp64"`` (default, the reference's double
-precision end to end) or ``"mixed"`` (AO, MO and Jastrow in FP32; Slater
-elimination, traces and energies in FP64).
+precision end to end) or ``"mixed"`` (psi, pdf, sweeps and drift with AO, MO and
+Jastrow in FP32, Slater elimination and acceptance in FP64).  Kinetic and local
+energies are FP64 in both.
 """
 from typing import Optional, Union
 
@@ -25,7 +26,8 @@ from .jastrows.elec_elec.jastrow_factor_electron_electron import JastrowFactorEl
 from .jastrows.elec_elec.kernels.pade_jastrow_kernel import PadeJastrowKernel
 from .orbitals.atomic_orbitals import AtomicOrbitals
 from .orbitals.molecular_orbitals import MolecularOrbitals
-from .plan import DevicePlan, SystemTables
+from .param_views import ParamViews, f64
+from .plan import DevicePlan, SystemTables, nuclear_repulsion
 from .pooling.orbital_configurations import OrbitalConfigurations
 from .pooling.slater_pooling import SlaterPooling
 from .wf_base import WaveFunction
@@ -51,18 +53,18 @@ class SlaterJastrow(WaveFunction):
         super().__init__(mol.nelec, 3, kinetic, cuda)
         if backflow is not None:
             raise NotImplementedError("backflow orbitals are not part of the MI355X path")
-        if kinetic != "jacobi":
-            raise NotImplementedError("kinetic='%s': the device path computes the kinetic energy by the Jacobi formula"
-                                      % kinetic)
+        if kinetic not in ("jacobi", "auto"):
+            raise ValueError("kinetic must be 'jacobi' or 'auto'")
         if not include_all_mo and isinstance(configs, str) and configs.startswith("cas("):
             raise ValueError("CAS calculation only possible with include_all_mo=True")
-        if precision not in _PREC or (eloc_precision is not None and eloc_precision not in _PREC):
+        if precision not in _PREC:
             raise ValueError("precision must be 'fp64' or 'mixed'")
-        # precision: psi, pdf and the Metropolis sweep; eloc_precision: kinetic and
-        # local energy (FP64 unless asked otherwise: FP32 AO values leave
-        # ~1e-5 Ha/electron errors on rare walkers, FP64 costs little there)
+        if eloc_precision not in (None, "fp64"):
+            raise ValueError("the kinetic and local energies are evaluated in FP64: FP32 atomic orbitals left "
+                             "errors above 1e-6 Ha/electron near nodes of psi, at no speed gain")
+        # precision: psi, pdf, the Metropolis / HMC sweeps and the drift
         self.precision = precision
-        self.eloc_precision = eloc_precision or "fp64"
+        self.eloc_precision = "fp64"
         self.mol = mol
         self.atoms = mol.atoms
         self.natom = mol.natom
@@ -107,11 +109,14 @@ class SlaterJastrow(WaveFunction):
             if cuda:
                 self.jastrow = self.jastrow.to(self.device)
 
+        # kinetic='auto' (wf_base.py:118-182: -1/2 lap psi / psi by double
+        # autograd) asks for the same quantity; the device evaluates it with
+        # the exact Jacobi kernel in either case
         self.kinetic_method = kinetic
         self.kinetic_energy = self.kinetic_energy_jacobi
         self.gradients = self.gradients_jacobi
         self._plan = None
-        self._plan_exp_version = None
+        self._plan_key = None
         self._plan_tables = None
         self.log_data()
 
@@ -132,63 +137,64 @@ class SlaterJastrow(WaveFunction):
         return ee, en
 
     def device_plan(self) -> DevicePlan:
-        """Plan for the current exponents (rebuilt if exponent sharing breaks)."""
-        ver = self.ao.bas_exp._version
-        if self._plan is not None and self._plan_exp_version == ver:
+        """Plan for the current exponents and atomic positions: rebuilt when
+        exponent sharing breaks or when ao.atom_coords moves (V_nn follows the
+        AO layer's coordinates, wf_base.py:97-116)."""
+        key = (self.ao.bas_exp._version, self.ao.atom_coords._version)
+        if self._plan is not None and self._plan_key == key:
             return self._plan
         ex = self.ao.bas_exp.detach().to("cpu", torch.float64).numpy()
-        if self._plan is not None and self._plan_tables.grouping_valid(ex):
-            self._plan_exp_version = ver
+        coords = self.ao.atom_coords.detach().to("cpu", torch.float64).numpy()
+        vnn = nuclear_repulsion(coords, self.mol.atomic_number)
+        if self._plan is not None and self._plan_tables.grouping_valid(ex) and vnn == self._plan_tables.vnn:
+            self._plan_key = key
             return self._plan
         ee, en = self._jastrow_terms()
         tab = SystemTables(
             self.mol.basis, self.mol.atom_coords, self.mol.atomic_number, self.mol.nup, self.mol.ndown,
             (self.configs[0].numpy(), self.configs[1].numpy()), self.highest_occ_mo,
-            ee is not None, en is not None, self.ao.norm_cst.cpu().numpy(), ex,
+            ee is not None, en is not None, self.ao.norm_cst.cpu().numpy(), ex, vnn_coords=coords,
         )
         self._plan = DevicePlan(tab)
         self._plan_tables = tab
-        self._plan_exp_version = ver
+        self._plan_key = key
         return self._plan
 
-    def _gather_index(self):
-        """Device index tensors of the current plan: exponent of every group
-        primitive (into bas_exp) and AO of every entry (into the rows of W)."""
-        plan = self.device_plan()
-        if getattr(self, "_gather_for", None) is not plan:
-            tab = self._plan_tables
-            dev = self.ao.bas_exp.device
-            self._gather = (torch.as_tensor(tab.grp_prim_src.astype(np.int64), device=dev),
-                            torch.as_tensor(tab.gao_mu.astype(np.int64), device=dev))
-            self._gather_for = plan
-        return self._gather
-
     def _params(self, precision: str):
-        """Parameter views handed to the kernels (all tiny device tensors)."""
+        """Parameter views handed to the kernels: one gather launch refreshes W
+        and the grouped exponents from the live parameters; the remaining
+        pointers are the parameters themselves (float64)."""
         ee, en = self._jastrow_terms()
-        f64 = torch.float64
-        W = self.mo.weight()[:, : self.highest_occ_mo]
-        W = W.detach().to(torch.float32 if precision == "mixed" else f64).contiguous()
-        bas_exp = self.ao.bas_exp.detach().to(f64).contiguous()
-        src, mu = self._gather_index()
-        gexp = bas_exp.index_select(0, src)
-        nb = self.device_plan().bucket
-        went = torch.zeros(len(mu), nb, dtype=W.dtype, device=W.device)
-        went[:, : W.shape[1]] = W.index_select(0, mu)
+        plan = self.device_plan()
+        mod = f64(self.mo.mo_modifier)
+        dev = mod.device
+        pv = getattr(self, "_pviews", None)
+        if pv is None or pv.plan is not plan or pv.mo_w64.device != dev:
+            pv = self._pviews = ParamViews(plan, dev)
+        mo_scf = self.mo.mo_scf  # a constant of the MO layer (not a parameter)
+        if mo_scf.device != dev or mo_scf.dtype != torch.float64 or not mo_scf.is_contiguous():
+            c = getattr(self, "_mo_scf64", None)
+            if c is None or c.device != dev or c.shape != mo_scf.shape:
+                c = self._mo_scf64 = mo_scf.to(dev, torch.float64).contiguous()
+            mo_scf = c
+        bas_exp = f64(self.ao.bas_exp)
+        pv.gather(mo_scf, mod, bas_exp)
+        mixed = precision == "mixed"
         keep = {
+            "mo_modifier": mod,
             "bas_exp": bas_exp,
-            "atom_coords": self.ao.atom_coords.detach().to(f64).contiguous(),
-            "mo_w": W,
-            "ci": self.fc.weight.detach().reshape(-1).to(f64).contiguous(),
-            "w_ee": ee.jastrow_kernel.weight.detach().to(f64).contiguous() if ee is not None else None,
-            "w_en": en.jastrow_kernel.weight.detach().to(f64).contiguous() if en is not None else None,
-            "grp_exp": gexp,
-            "grp_expf": gexp.to(torch.float32),
-            "entry_w": went,
+            "atom_coords": f64(self.ao.atom_coords),
+            "ci": f64(self.fc.weight).reshape(-1),
+            "w_ee": f64(ee.jastrow_kernel.weight) if ee is not None else None,
+            "w_en": f64(en.jastrow_kernel.weight) if en is not None else None,
+            "views": pv,
         }
         p = N.QmcParams()
-        for k, v in keep.items():
-            setattr(p, k, N.ptr(v))
+        p.bas_exp, p.atom_coords = bas_exp.data_ptr(), keep["atom_coords"].data_ptr()
+        p.ci, p.w_ee, p.w_en = keep["ci"].data_ptr(), N.ptr(keep["w_ee"]), N.ptr(keep["w_en"])
+        p.mo_w = (pv.mo_w32 if mixed else pv.mo_w64).data_ptr()
+        p.entry_w = (pv.entry_w32 if mixed else pv.entry_w64).data_ptr()
+        p.grp_exp, p.grp_expf = pv.grp_exp.data_ptr(), pv.grp_expf.data_ptr()
         return p, keep
 
     def _check_pos(self, pos: torch.Tensor) -> torch.Tensor:
@@ -245,6 +251,10 @@ class SlaterJastrow(WaveFunction):
         out = self._eval(pos, N.QMC_EVAL_PSI, precision)
         return out["logabs"], out["sign"]
 
+    def kinetic_energy_autograd(self, x: torch.Tensor) -> torch.Tensor:
+        """-1/2 lap(psi)/psi (wf_base.py:118-182), evaluated exactly by the Jacobi kernel."""
+        return self.kinetic_energy_jacobi(x)
+
     def kinetic_energy_jacobi(self, x: torch.Tensor, **kwargs) -> torch.Tensor:
         """-1/2 lap(psi)/psi by the Jacobi formula, (Nbatch, 1) (slater_jastrow.py:312-344)."""
         return self._eval(x, N.QMC_EVAL_ELOC)["kinetic"].to(torch.get_default_dtype()).view(-1, 1)
@@ -257,9 +267,32 @@ class SlaterJastrow(WaveFunction):
         """All outputs of one local-energy evaluation (float64 dict)."""
         return self._eval(pos, N.QMC_EVAL_ELOC, precision)
 
-    def nuclear_repulsion(self) -> float:
-        """V_nn (wf_base.py:97-116)."""
-        return self.device_plan().tables.vnn
+    def nuclear_repulsion(self) -> torch.Tensor:
+        """V_nn of the AO layer's atomic positions (wf_base.py:97-116), a 0-d tensor."""
+        return torch.as_tensor(self.device_plan().tables.vnn, dtype=torch.get_default_dtype())
+
+    def _potentials(self, pos: torch.Tensor):
+        x = self._check_pos(pos)
+        nw = x.shape[0]
+        vee = torch.empty(nw, dtype=torch.float64, device=x.device)
+        ven = torch.empty(nw, dtype=torch.float64, device=x.device)
+        if nw:
+            prm, keep = self._params("fp64")
+            o = N.QmcJastrowOut()
+            o.v_ee, o.v_en = vee.data_ptr(), ven.data_ptr()
+            N.check(N.load().qmc_jastrow_eval(self.device_plan().handle, prm, 0, x.data_ptr(), nw, o,
+                                              N.current_stream()))
+        dt = torch.get_default_dtype()
+        return vee.to(dt).view(-1, 1), ven.to(dt).view(-1, 1)
+
+    def electronic_potential(self, pos: torch.Tensor) -> torch.Tensor:
+        """sum_{i<j} 1/r_ij per walker, (Nbatch, 1) (wf_base.py:49-70)."""
+        return self._potentials(pos)[0]
+
+    def nuclear_potential(self, pos: torch.Tensor) -> torch.Tensor:
+        """-sum_{i,A} Z_A / r_iA per walker with the AO layer's atomic positions,
+        (Nbatch, 1) (wf_base.py:72-95)."""
+        return self._potentials(pos)[1]
 
     def local_energy_autodiff(self, pos: torch.Tensor) -> torch.Tensor:
         """Local energy differentiable with respect to the parameters through the
@@ -322,15 +355,13 @@ class SlaterJastrow(WaveFunction):
 class _PsiFunction(torch.autograd.Function):
     """psi(pos; theta) with a hand-written adjoint for theta.
 
-    backward: with g_w = dL/dpsi_w * psi_w and the adjoint rows of
-    qmc_logpsi_grad_rows,
-        dL/dW       = AO_rows^T (g * Gamma_rows)       (one GEMM)
-        dL/dmo_mod  = dL/dW * mo_scf                   (molecular_orbitals.py:92)
-        dL/dc       = g^T (D_n / S)
-        dL/dw_ee/en = g^T djas
-        dL/dbas_exp = g^T dexp_rows
-    AO values are recomputed in the backward, nothing is stored by the forward.
-    Gradients with respect to the positions are not produced."""
+    backward: with g_w = dL/dpsi_w * psi_w, one qmc_logpsi_grad launch pair
+    (AO values recomputed, per-wavefront partial sums, one slot reduction)
+    gives dL/dW = sum_w g_w AO_w^T Gamma_w, dL/dc, dL/dw_ee, dL/dw_en and
+    dL/dbas_exp; then dL/dmo_modifier = dL/dW * mo_scf (molecular_orbitals.py:92).
+    Nothing is stored by the forward.  With respect to the positions:
+    dL/dx = dL/dpsi psi grad ln|psi| (the FP64 drift kernel, the derivative of
+    gradients_jacobi)."""
 
     @staticmethod
     def forward(ctx, wf, x, mo_mod, ci, bas_exp, w_ee, w_en):
@@ -345,20 +376,73 @@ class _PsiFunction(torch.autograd.Function):
         wf = ctx.wf
         x, psi = ctx.saved_tensors
         g = grad_out.reshape(-1).to(torch.float64) * psi
-        rows = wf_adjoint_rows(wf, x, need_exp=ctx.needs_input_grad[4])
-        nel = wf.nelec
-        gr = g.repeat_interleave(nel)
-        dW = rows["ao"].T @ (rows["gamma"] * gr[:, None])  # (nao, nmo_used)
+        d_x = None
+        if ctx.needs_input_grad[1]:
+            dr, _, _ = wf.drift(x, "fp64")
+            d_x = (g[:, None] * dr.view(x.shape[0], -1)).to(x.dtype)
+        if not any(ctx.needs_input_grad[2:]):
+            return None, d_x, None, None, None, None, None
+        r = adjoint_reduce(wf, x, g, "logpsi", need_exp=ctx.needs_input_grad[4])
+        return (None, d_x) + _param_grads(wf, r, ctx.needs_input_grad)
+
+
+def _param_grads(wf, r, needs):
+    """Gradients of (mo_modifier, fc.weight, bas_exp, w_ee, w_en) from the
+    reduced adjoint outputs."""
+    dt = torch.get_default_dtype()
+    d_mod = d_ci = d_exp = d_ee = d_en = None
+    if needs[2]:
         full = torch.zeros_like(wf.mo.mo_modifier, dtype=torch.float64)
-        full[:, : dW.shape[1]] = dW
-        d_mod = (full * wf.mo.mo_scf.to(full)).to(wf.mo.mo_modifier.dtype) if ctx.needs_input_grad[2] else None
-        d_ci = (g @ rows["dci"]).view(1, -1).to(wf.fc.weight.dtype) if ctx.needs_input_grad[3] else None
-        d_exp = None
-        if ctx.needs_input_grad[4]:
-            d_exp = (gr @ rows["dexp"]).to(wf.ao.bas_exp.dtype)
-        d_ee = (g @ rows["djas"][:, 0]).view(1).to(torch.get_default_dtype()) if ctx.needs_input_grad[5] else None
-        d_en = (g @ rows["djas"][:, 1]).view(1).to(torch.get_default_dtype()) if ctx.needs_input_grad[6] else None
-        return None, None, d_mod, d_ci, d_exp, d_ee, d_en
+        full[:, : r["w"].shape[1]] = r["w"]
+        d_mod = (full * wf.mo.mo_scf.to(full)).to(wf.mo.mo_modifier.dtype)
+    if needs[3]:
+        d_ci = r["ci"].view(1, -1).to(wf.fc.weight.dtype)
+    if needs[4] and r.get("exp") is not None:
+        d_exp = r["exp"].to(wf.ao.bas_exp.dtype)
+    if needs[5]:
+        d_ee = r["jas"][0:1].to(dt)
+    if needs[6]:
+        d_en = r["jas"][1:2].to(dt)
+    return d_mod, d_ci, d_exp, d_ee, d_en
+
+
+def adjoint_reduce(wf: SlaterJastrow, pos: torch.Tensor, weights: torch.Tensor, which: str,
+                   need_exp: bool = False, workspace_bytes: Optional[int] = None):
+    """sum_w g_w d f_w / d theta for f = log|psi| (which="logpsi", qmc_logpsi_grad)
+    or f = E_L (which="eloc", qmc_eloc_grad): dict of w (nao, nmo), ci (nci,),
+    jas (2,), exp (nprim) or None, and eloc (nw,) for E_L.  The workspace is
+    the size that uses the most partial-sum slots unless workspace_bytes is given."""
+    x = wf._check_pos(pos)
+    nw = x.shape[0]
+    plan = wf.device_plan()
+    prm, keep = wf._params("fp64")
+    dev, f64_ = x.device, torch.float64
+    sz = plan.sizes()
+    lib = N.load()
+    with_exp = need_exp and which == "logpsi"
+    out = {
+        "w": torch.empty(sz.nao, sz.nmo, dtype=f64_, device=dev),
+        "ci": torch.empty(plan.tables.nci, dtype=f64_, device=dev),
+        "jas": torch.empty(2, dtype=f64_, device=dev),
+        "exp": torch.empty(plan.tables.nprim_ref, dtype=f64_, device=dev) if with_exp else None,
+        "eloc": torch.empty(nw, dtype=f64_, device=dev) if which == "eloc" else None,
+    }
+    if workspace_bytes is None:
+        workspace_bytes = int(lib.qmc_adjoint_workspace_size(plan.handle, nw, 1 if with_exp else 0))
+    ws = torch.empty(max(workspace_bytes, 8), dtype=torch.uint8, device=dev)
+    g = weights.detach().to(dev, f64_).contiguous()
+    o = N.QmcGradOut()
+    o.grad_w, o.grad_ci, o.grad_jas, o.grad_exp = (out["w"].data_ptr(), out["ci"].data_ptr(),
+                                                    out["jas"].data_ptr(), N.ptr(out["exp"]))
+    xp = x.data_ptr() if nw else 0
+    gp = g.data_ptr() if nw else 0
+    if which == "logpsi":
+        N.check(lib.qmc_logpsi_grad(plan.handle, prm, xp, nw, gp, o, ws.data_ptr(), workspace_bytes,
+                                    N.current_stream()))
+    else:
+        N.check(lib.qmc_eloc_grad(plan.handle, prm, xp, nw, gp, N.ptr(out["eloc"]) if nw else 0, o,
+                                  ws.data_ptr(), workspace_bytes, N.current_stream()))
+    return out
 
 
 def wf_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_exp: bool = False):
@@ -377,6 +461,8 @@ def wf_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_exp: bool = False
         "djas": torch.empty(nw, 2, dtype=f64, device=dev),
         "dexp": torch.empty(nw * nel, plan.tables.nprim_ref, dtype=f64, device=dev) if need_exp else None,
     }
+    if nw == 0:  # empty rows reduce to zero gradients
+        return rows
     N.check(N.load().qmc_logpsi_grad_rows(plan.handle, prm, x.data_ptr(), nw, rows["ao"].data_ptr(),
                                           rows["gamma"].data_ptr(), rows["dci"].data_ptr(),
                                           rows["djas"].data_ptr(), N.ptr(rows["dexp"]), N.current_stream()))
@@ -384,10 +470,10 @@ def wf_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_exp: bool = False
 
 
 class _ELocFunction(torch.autograd.Function):
-    """E_L(pos; theta) with the hand-written adjoint of qmc_eloc_grad_rows:
-        dL/dW    = ao_rows^T (g * ga_rows) + b_rows^T (g * gb_rows)
-        dL/dc    = g^T dci,   dL/dw_ee,en = g^T djas
-    (g = dL/dE_L per walker).  Exponent gradients of E_L are not provided."""
+    """E_L(pos; theta) with the hand-written adjoint of qmc_eloc_grad: for
+    g = dL/dE_L per walker, dL/dW = sum_w g_w (AO_w^T GA_w + b_w^T GB_w),
+    dL/dc = g^T dE_L/dc, dL/dw_ee,en = g^T dE_L/dw (reduced on the device).
+    Exponent and position gradients of E_L are not provided."""
 
     @staticmethod
     def forward(ctx, wf, x, mo_mod, ci, bas_exp, w_ee, w_en):
@@ -403,22 +489,11 @@ class _ELocFunction(torch.autograd.Function):
         if ctx.needs_input_grad[4]:
             raise NotImplementedError("d E_L / d bas_exp is not provided by the device adjoint: "
                                       "freeze 'ao' (Solver.configure(freeze=['ao']))")
+        # no gradient with respect to the positions: sampled positions arrive
+        # with requires_grad set (metropolis.py:177) but are constants here
         g = grad_out.reshape(-1).to(torch.float64)
-        r = eloc_adjoint_rows(wf, x, need_rows=ctx.needs_input_grad[2])
-        gr = g.repeat_interleave(wf.nelec)
-        d_mod = d_ci = d_ee = d_en = None
-        if ctx.needs_input_grad[2]:
-            dW = r["ao"].T @ (r["ga"] * gr[:, None]) + r["b"].T @ (r["gb"] * gr[:, None])
-            full = torch.zeros_like(wf.mo.mo_modifier, dtype=torch.float64)
-            full[:, : dW.shape[1]] = dW
-            d_mod = (full * wf.mo.mo_scf.to(full)).to(wf.mo.mo_modifier.dtype)
-        if ctx.needs_input_grad[3]:
-            d_ci = (g @ r["dci"]).view(1, -1).to(wf.fc.weight.dtype)
-        if ctx.needs_input_grad[5]:
-            d_ee = (g @ r["djas"][:, 0]).view(1).to(torch.get_default_dtype())
-        if ctx.needs_input_grad[6]:
-            d_en = (g @ r["djas"][:, 1]).view(1).to(torch.get_default_dtype())
-        return None, None, d_mod, d_ci, None, d_ee, d_en
+        r = adjoint_reduce(wf, x, g, "eloc")
+        return (None, None) + _param_grads(wf, r, ctx.needs_input_grad)
 
 
 def eloc_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_rows: bool = True):
@@ -439,6 +514,8 @@ def eloc_adjoint_rows(wf: SlaterJastrow, pos: torch.Tensor, need_rows: bool = Tr
         "djas": torch.empty(nw, 2, dtype=f64, device=dev),
         "drift": torch.empty(nw, nel, 3, dtype=f64, device=dev),
     }
+    if nw == 0:
+        return r
     N.check(N.load().qmc_eloc_grad_rows(plan.handle, prm, x.data_ptr(), nw, N.ptr(r["eloc"]), N.ptr(r["ao"]),
                                         N.ptr(r["b"]), N.ptr(r["ga"]), N.ptr(r["gb"]), N.ptr(r["dci"]),
                                         N.ptr(r["djas"]), N.ptr(r["drift"]), N.current_stream()))

@@ -110,9 +110,3 @@ def test_one_electron_energy_matches_all_electron():
     _say("  one-elec %.5f  all-elec %.5f  se %.5f" % (e1.mean(), ea.mean(), se))
     assert abs(float(e1.mean() - ea.mean())) < 5 * se
 
-
-def test_all_electron_iterated_moves_raise():
-    mol = systems.h2()
-    wf, _ = helpers.make_pair(mol, w_ee=1.0, precision="mixed")
-    with pytest.raises(NotImplementedError):
-        _metropolis(mol, 8, 2, "all-elec-iter", seed=1)(wf.pdf, with_tqdm=False)

@@ -192,3 +192,73 @@ def test_server_realtime_poisson(contexts):
     _say("  ", s)
     assert s["requests"] == 40 and all(r.done >= r.arrival for r in res)
     assert all(np.isfinite(r.energy) for r in res)
+
+
+def test_config4_sixty_four_uneven_systems(contexts):
+    """The seeded 64-system draw of the batched-uneven configuration (systems from
+    {H2, LiH, H2O, N2, C6H6}, walkers log-uniform 256-8192, 172 650 walkers):
+    one batched sweep per chunk equals one launch per system bit for bit, and
+    the FP64 batched local energy matches the oracle on the first walkers of
+    every system within 1e-10 relative."""
+    draw = catalog.uneven_batch(64, seed=0)
+    assert min(n for _, n in draw) >= 256 and max(n for _, n in draw) <= 8192 and len(draw) == 64
+    _say("case config 4: 64 systems, %d walkers" % sum(n for _, n in draw))
+    jobs = [(name, nw, 500 + i) for i, (name, nw) in enumerate(draw)]
+    a = _walker_sets(contexts, jobs)
+    b = _walker_sets(contexts, jobs)
+    runner = BatchRunner(64)
+    for nstep in (7, 13):
+        runner.sweep([(w, nstep) for w in a])
+        sweep_each([(w, nstep) for w in b])
+    torch.cuda.synchronize()
+    same = sum(_same(x, y) for x, y in zip(a, b))
+    _say("  identical walker sets after 20 steps: %d / 64" % same)
+    assert same == 64
+    el = [torch.empty(w.nw, dtype=torch.float64, device="cuda") for w in a]
+    runner.local_energy(list(zip(a, el)))
+    torch.cuda.synchronize()
+    orcs = {}
+    worst = 0.0
+    for (name, nw, _), w, e in zip(jobs, a, el):
+        mol = contexts[name].wf.mol
+        if name not in orcs:
+            wf = contexts[name].wf
+            ee, en = wf._jastrow_terms()
+            orcs[name] = O.OracleSlaterJastrow(mol.basis, mol.atom_coords, mol.atomic_number, mol.nup, mol.ndown,
+                                               jastrow_ee=float(ee.jastrow_kernel.weight),
+                                               jastrow_en=float(en.jastrow_kernel.weight))
+        k = 2 if name == "c6h6" else 8
+        ref = orcs[name].local_energy(w.pos[:k].cpu()).detach().view(-1)
+        got = e[:k].cpu()
+        worst = max(worst, float(((got - ref).abs() / ref.abs()).max()))
+    _say("  FP64 batched E_L vs oracle: max rel err %.2e" % worst)
+    assert worst < 1e-10
+
+
+def test_config5_serving_at_full_load(contexts):
+    """Poisson arrivals at 2000 req/s of the serving configuration (molecules of
+    the config-4 set, walkers log-uniform 128-16384, 1-20 sweeps + one E_L
+    batch), in real time: every request completes, a sample of requests gives
+    the same result alone as in the mix, and the plain per-request path agrees
+    within 1e-12 relative."""
+    import time
+
+    reqs = poisson_requests(rate=2000.0, n=120, seed=11, wmax=16384)
+    assert max(r.nwalkers for r in reqs) > 8192
+    _say("case config 5: 120 requests at 2000 req/s, walkers up to %d" % max(r.nwalkers for r in reqs))
+    server = VMCServer(precision="mixed", steps_per_tick=4, contexts=contexts)
+    server.run(reqs[:2], realtime=False)  # warm every instance used
+    t0 = time.perf_counter()
+    res = server.run(reqs, realtime=True)
+    s = latency_summary(res, time.perf_counter() - t0)
+    _say("  ", {k: round(v, 3) if isinstance(v, float) else v for k, v in s.items()})
+    assert s["requests"] == 120 and server.peak_active > 1
+    mix = {r.rid: r for r in res}
+    for req in reqs[::17]:
+        alone = server.run([req], realtime=False)[0]
+        assert np.array_equal(alone.eloc, mix[req.rid].eloc), req
+    sample = reqs[::10]
+    plain = {r.rid: r for r in serve_per_request(sample, contexts, realtime=False)}
+    worst = max(float(np.max(np.abs(plain[k].eloc - mix[k].eloc) / np.abs(mix[k].eloc))) for k in plain)
+    _say("  plain per-request path vs server on %d requests: max rel |dE_L| %.2e" % (len(plain), worst))
+    assert worst < 1e-12

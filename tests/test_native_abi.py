@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "qmc_mi355x.h")
 def _declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qmc_\w+)\s*\(", src, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|size_t)\s+(qmc_\w+)\s*\(", src, flags=re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -50,6 +50,12 @@ def test_ctypes_struct_layout_matches_header(tmp_path):
         "QmcHmcDesc": N.QmcHmcDesc,
         "QmcSweepJob": N.QmcSweepJob,
         "QmcEvalJob": N.QmcEvalJob,
+        "QmcParamViews": N.QmcParamViews,
+        "QmcPlanSizes": N.QmcPlanSizes,
+        "QmcJastrowOut": N.QmcJastrowOut,
+        "QmcAoOut": N.QmcAoOut,
+        "QmcWalkerInit": N.QmcWalkerInit,
+        "QmcGradOut": N.QmcGradOut,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "qmc_mi355x.h"', "int main(void) {"]
     for sname, cls in structs.items():
@@ -156,6 +162,7 @@ def _corrupt(tab, field, index, value):
     ("grp_prim_src", 3, 10 ** 6, "grp_prim_src"),
     ("grp_nao", 0, 10 ** 4, "AO entries"),
     ("grp_ao0", 1, -3, "AO entries"),
+    ("grp_ao0", 0, 1, "AO entries"),
     ("gao_mu", 2, 11, "gao_mu"),
     ("gao_src", 5, -1, "gao_src"),
     ("gao_k", 0, 4, "gao_k"),
