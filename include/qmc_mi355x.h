@@ -316,6 +316,22 @@ int qmc_eloc_grad(const QmcPlan* plan, const QmcParams* params, const double* po
                   qmc_stream_t stream);
 
 /*
+ * qmc_logpsi_grad without a workspace argument: the partial-sum slots live in
+ * a static device array of QMC_NOALLOC_SCRATCH_BYTES per GPU (allocated when
+ * the library is loaded, held until the process ends), so the call allocates
+ * nothing -- for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  Returns QMC_ERR_UNSUPPORTED when one slot of the system
+ * exceeds the array.  Every call on one GPU shares that GPU's array: calls
+ * that overlap in time on one GPU race (order them on one stream).  Results
+ * equal qmc_logpsi_grad's with a workspace of QMC_NOALLOC_SCRATCH_BYTES bit
+ * for bit (same slot count, same summation order).
+ */
+#define QMC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int qmc_logpsi_grad_noalloc(const QmcPlan* plan, const QmcParams* params, const double* pos, int64_t nw,
+                            const double* weights, QmcGradOut* out, qmc_stream_t stream);
+
+/*
  * Batched launches: many systems (plans), each with its own parameters,
  * walkers, walker count and step count, in one launch per register bucket
  * (occupied orbitals per spin <= 2, 4, 8, 16, 24) and radial type.  Waves of

@@ -133,6 +133,7 @@ class QmcGradOut(ctypes.Structure):
 
 QMC_INIT_CENTER, QMC_INIT_UNIFORM, QMC_INIT_NORMAL, QMC_INIT_ATOMIC = 0, 1, 2, 3
 QMC_INIT_MAXATOM = 32
+QMC_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class QmcWalkerInit(ctypes.Structure):
@@ -189,6 +190,8 @@ EXPORTS = {
     "qmc_eloc_grad": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                        ctypes.c_void_p, ctypes.POINTER(QmcGradOut), ctypes.c_void_p, ctypes.c_size_t,
                        ctypes.c_void_p], ctypes.c_int),
+    "qmc_logpsi_grad_noalloc": ([ctypes.c_void_p, ctypes.POINTER(QmcParams), ctypes.c_void_p, ctypes.c_int64,
+                                 ctypes.c_void_p, ctypes.POINTER(QmcGradOut), ctypes.c_void_p], ctypes.c_int),
     "qmc_walkers_init": ([ctypes.POINTER(QmcWalkerInit), ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p,
                           ctypes.c_void_p], ctypes.c_int),
     "qmc_last_error": ([], ctypes.c_char_p),

@@ -25,7 +25,7 @@ _FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep
 ]
 # HMC / drift kernels: single-configuration (MC=1) and multi-configuration (MC=8) instances
 _HMC = [("qmc_hmc_f32.hip", mc) for mc in (1, 8)] + [("qmc_hmc_f64.hip", mc) for mc in (1, 8)]
-UNITS = [("qmc_abi.hip", "qmc_abi", []), ("qmc_api.hip", "qmc_api", [])] + [
+UNITS = [("qmc_abi.hip", "qmc_abi", []), ("qmc_api.hip", "qmc_api", []), ("qmc_noalloc.hip", "qmc_noalloc", [])] + [
     (f, f.replace(".hip", "_r%d" % r), ["-DQMC_RAD=%d" % r]) for f in _FAMILIES for r in (0, 1)
 ] + [(f, f.replace(".hip", "_mc%d_r%d" % (mc, r)), ["-DQMC_RAD=%d" % r, "-DQMC_HMC_MC=%d" % mc])
      for f, mc in _HMC for r in (0, 1)]
