@@ -556,7 +556,8 @@ def main():
 
     # the table-driven kernels (any basis layout) on the same workload and launch length
     table = None
-    if plan.specialization is not None:
+    spec_name = plan.specialization
+    if spec_name is not None:
         plan.set_specialized(False)
         try:
             tl = launches(0, max(args.steps, spl))
@@ -570,7 +571,7 @@ def main():
             table = {"walker_steps_per_s_device": nw * nst / tdev, "launch_ms": tdev / len(tl) * 1e3,
                      "steps_per_launch": tl[0][1], "frac": (F + 4 * T) * nw * nst / tdev / 1e12 / PEAK_FP32_TFLOPS,
                      "note": "table-driven AO rows (any basis layout); the headline uses the rows generated "
-                             "for this layout (%s)" % plan.specialization}
+                             "for this layout (%s)" % spec_name}
         finally:
             plan.set_specialized(True)
 
@@ -626,7 +627,7 @@ def main():
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "kernel": "sweep_kernel_wlds" if plan.specialization else "sweep_kernel",
+            "kernel": "sweep_kernel_wlds" if spec_name else "sweep_kernel",
             "launch_ms": float(np.mean(launch_s)) * 1e3,
             "launches": len(launch_s),
             "flops_per_walker_step": F + 4 * T,

@@ -124,6 +124,105 @@ struct MoSink {
     }
 };
 
+// AO -> MO contraction on the matrix cores for the large register buckets
+// (molecular_orbitals.py:79-95): MO = AO W with the wavefront's 64 lanes as the
+// rows of AO (one electron each), the AO entries as K and the bucket's columns
+// as N, by v_mfma_f32_32x32x2f32 (FP32 products, FP32 accumulation) in two
+// 32-row tiles.  AO entries arrive one at a time and are issued in pairs: for
+// entries (i, j) one v_permlane32_swap turns the lanes' values a_i, a_j
+// into the A operands of the two tiles (lanes 0-31 hold k = 0 = entry i,
+// lanes 32-63 k = 1 = entry j, of rows 0-31 and of rows 32-63), and lane
+// c + 32 k loads B[k][c] = W[k ? j : i][c].  finish() moves the accumulators
+// (lane l holds column l mod 32 of rows (r mod 4) + 8 (r / 4) + 4 (l / 32),
+// r < 16) into each lane's MO row through the wave's row buffer (the LDS the
+// Slater elimination uses next; 64 rows of NM + 1 floats fit in its 64 NM
+// doubles).
+typedef float qmc_floatx16 __attribute__((ext_vector_type(16)));
+
+template <int NM>
+__device__ __forceinline__ double* wave_rowbuf();
+
+template <int NM, bool WLDS>
+struct MfmaSink {
+    static_assert(NM <= 32, "one 32-column tile");
+    const float* W;  // entry-ordered W rows, stride NM (LDS copy with WLDS)
+    int nentry;
+    int lane;
+    float pend;
+    int pend_i;
+    bool have;
+    qmc_floatx16 acc0, acc1;
+
+    __device__ __forceinline__ void init(const float* w, int n) {
+        W = w;
+        nentry = n;
+        lane = threadIdx.x & 63;
+        have = false;
+        pend = 0.0f;
+        pend_i = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0.0f;
+    }
+    // generated rows call fence() per exponent group: the W pointer becomes an
+    // opaque value depending on the pending AO value (not on the accumulators,
+    // which would wait for the last MFMA), so B loads are issued where their
+    // group is reached rather than hoisted over the whole step loop
+    __device__ __forceinline__ void fence() {
+        if constexpr (WLDS) {
+            const QMC_LDS float* w = (const QMC_LDS float*)W;
+            asm volatile("" : "+s"(w) : "v"(pend));
+            W = (const float*)w;
+        }
+    }
+    // entries i0 and i1 (i1 < 0: none) as k = 0 and k = 1 of one MFMA pair
+    __device__ __forceinline__ void issue(int i0, int i1, float a0, float a1) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a0), __float_as_uint(a1), false, false);
+        const float A0 = __uint_as_float(r[0]), A1 = __uint_as_float(r[1]);
+        const int col = lane & 31;
+        const int ik = (lane >> 5) ? i1 : i0;
+        float b = 0.0f;
+        if (col < NM && ik >= 0 && ik < nentry) {
+            if constexpr (WLDS)
+                b = ((const QMC_LDS float*)W)[ik * NM + col];
+            else
+                b = W[ik * NM + col];
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1, b, acc1, 0, 0, 0);
+    }
+    template <typename V>
+    __device__ __forceinline__ void operator()(int i, int, V ao, V, V, V, V) {
+        if (!have) {
+            pend = (float)ao;
+            pend_i = i;
+            have = true;
+        } else {
+            issue(pend_i, i, pend, (float)ao);
+            have = false;
+        }
+    }
+    __device__ __forceinline__ void finish(float (&mo)[NM]) {
+        if (have) issue(pend_i, -1, pend, 0.0f);
+        have = false;
+        constexpr int S = NM + 1;  // odd row stride: conflict-free column reads
+        float* T = (float*)wave_rowbuf<NM>();
+        wave_lds_sync();  // the previous elimination's reads of the buffer are done
+        const int col = lane & 31, rb = 4 * (lane >> 5);
+        if (col < NM) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = (r & 3) + 8 * (r >> 2) + rb;
+                T[row * S + col] = acc0[r];
+                T[(32 + row) * S + col] = acc1[r];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int m = 0; m < NM; ++m) mo[m] = T[lane * S + m];
+        wave_lds_sync();  // the tile is read before the elimination stores rows
+    }
+};
+
 // Pade Jastrow terms of one electron (jastrow_factor_electron_electron.py:124-260,
 // jastrow_factor_electron_nuclei.py:56-161, pade kernels): value contributions,
 // gradient of ln J with respect to this electron and its Laplacian of ln J;
@@ -656,11 +755,20 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
                                               double y, double z, double logc0, double& sgn,
                                               const T* wlds = nullptr) {
     T mo[NM];
-#pragma unroll
-    for (int m = 0; m < NM; ++m) mo[m] = T(0);
     const T* w = WLDS ? wlds : (const T*)prm.entry_w;
-    MoSink<T, NM, false, T, false, WLDS> sink{w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
-    AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
+    // FP32 buckets of 16 and 24 columns contract on the matrix cores (a
+    // 64 x nentry x NM product per wavefront); smaller ones in VALU FMAs
+    if constexpr (std::is_same<T, float>::value && NM >= 16 && !EXC) {
+        MfmaSink<NM, WLDS> sink;
+        sink.init((const float*)w, t.nentry);
+        AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
+        sink.finish(mo);
+    } else {
+#pragma unroll
+        for (int m = 0; m < NM; ++m) mo[m] = T(0);
+        MoSink<T, NM, false, T, false, WLDS> sink{w, t.nmo, mo, nullptr, nullptr, nullptr, nullptr};
+        AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
+    }
     const T kj = jastrow_value<T>(t, prm, L, x, y, z);
     double logabs;
     if constexpr (EXC) {
@@ -750,10 +858,18 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         double sgn_n;
         double lpn = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
         if (lpn < lp_zero) lpn = lp_eps;
-        // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U
+        // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U; the
+        // logarithm in the bulk precision (the hardware log2 in the mixed sweep,
+        // the FP64 log in the FP64 sweep)
         const double U = u01_closed0(ubits);
-        const bool acc =
-            2.0 * (lpn - lp) >= (U > 0.0 ? 0.6931471805599453 * (double)__builtin_amdgcn_logf((float)U) : -INFINITY);
+        double logu = -INFINITY;
+        if (U > 0.0) {
+            if constexpr (std::is_same<T, double>::value)
+                logu = log(U);
+            else
+                logu = 0.6931471805599453 * (double)__builtin_amdgcn_logf((float)U);
+        }
+        const bool acc = 2.0 * (lpn - lp) >= logu;
         if (acc) {
             x = xn;
             y = yn;
