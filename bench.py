@@ -309,9 +309,9 @@ def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0
     for rate in rates:
         reqs = poisson_requests(rate, nreq, seed=seed)
         t0 = time.perf_counter()
-        res = server.run(reqs, realtime=True)
+        res = server.run(reqs, realtime=True, profile_ticks=True)
         row = {"rate_req_per_s": rate, "batched": latency_summary(res, time.perf_counter() - t0),
-               "peak_requests_in_flight": server.peak_active}
+               "peak_requests_in_flight": server.peak_active, "ticks": server.tick_profile}
         if plain:
             t0 = time.perf_counter()
             res_p = serve_per_request(reqs, ctx, realtime=True)
@@ -541,11 +541,19 @@ def main():
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in timed]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run(timed, False, events)
+    run(timed, False)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     value = nw * args.steps / elapsed
     acc = float(nacc.sum().item()) / (nw * args.steps)
+    # device time: launches of the same lengths, queued behind a short untimed
+    # launch so that every event pair brackets its kernel alone (on an idle GPU
+    # a start event completes before the host has submitted the launch, and
+    # the pair would also hold the submission latency)
+    s_dev = timed[-1][0] + timed[-1][1]
+    run([(s_dev, 5)], False)
+    run([(s_dev + 5 + s0 - timed[0][0], n) for s0, n in timed], False, events)
+    torch.cuda.synchronize()
     launch_s = [a.elapsed_time(b) * 1e-3 for a, b in events]
     device_s = float(np.sum(launch_s))
     steps_per_launch = timed[0][1]
@@ -560,10 +568,11 @@ def main():
     if spec_name is not None:
         plan.set_specialized(False)
         try:
-            tl = launches(0, max(args.steps, spl))
+            tl = launches(s_dev, args.steps)
             tev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in tl]
             run(launches(0, steps_per_launch), False)  # first launch of this instance
             torch.cuda.synchronize()
+            run([(s_dev, 5)], False)  # the timed launches queue behind it, as above
             run(tl, False, tev)
             torch.cuda.synchronize()
             tdev = float(np.sum([a.elapsed_time(b) * 1e-3 for a, b in tev]))

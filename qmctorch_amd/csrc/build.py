@@ -11,6 +11,7 @@ import argparse
 import os
 import re
 import subprocess
+import time
 import sys
 from concurrent.futures import ThreadPoolExecutor
 
@@ -21,7 +22,7 @@ LIB = os.path.join(PKG, "libqmcmi355x.so")
 # (source, object tag, extra defines): kernel families are compiled once per radial type
 _FAMILIES = ["qmc_wf_f32.hip", "qmc_wf_f64.hip", "qmc_sweep_f32.hip", "qmc_sweep_f64.hip", "qmc_grad.hip",
              "qmc_batch_sweep_f32.hip", "qmc_batch_sweep_f64.hip", "qmc_batch_wf_f32.hip", "qmc_batch_wf_f64.hip",
-             "qmc_exc.hip",
+             "qmc_exc.hip", "qmc_exc_hmc.hip",
 ]
 # HMC / drift kernels: single-configuration (MC=1) and multi-configuration (MC=8) instances
 _HMC = [("qmc_hmc_f32.hip", mc) for mc in (1, 8)] + [("qmc_hmc_f64.hip", mc) for mc in (1, 8)]
@@ -50,10 +51,22 @@ def _obj_dir() -> str:
 
 
 def compile_one(unit, extra=()) -> subprocess.CompletedProcess:
+    """Compile one unit into a temporary object, moved into place on success
+    and dated to the start of the compile: a source edited while hipcc ran
+    (its host and device passes read the sources at different times) leaves
+    the object older than the edit, so an incremental build recompiles it."""
     src, tag, defs = unit
     obj = os.path.join(_obj_dir(), tag + ".o")
-    cmd = [HIPCC, *FLAGS, *defs, *extra, "-c", src if os.path.isabs(src) else os.path.join(HERE, src), "-o", obj]
-    return subprocess.run(cmd, capture_output=True, text=True)
+    tmp = os.path.join(_obj_dir(), "%s.tmp%d.o" % (tag, os.getpid()))
+    t0 = time.time()
+    cmd = [HIPCC, *FLAGS, *defs, *extra, "-c", src if os.path.isabs(src) else os.path.join(HERE, src), "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode == 0:
+        os.replace(tmp, obj)
+        os.utime(obj, (t0, t0))
+    elif os.path.exists(tmp):
+        os.remove(tmp)
+    return r
 
 
 def spec_units():

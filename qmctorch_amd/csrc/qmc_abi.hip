@@ -33,6 +33,7 @@ int fail(int code, const std::string& msg) { return qmc::set_error(code, msg); }
     } while (0)
 
 const char* kNoBucket = "more than 24 occupied orbitals per spin";
+const char* kNoExcBucket = "the excitation path supports up to 16 MO columns per electron";
 
 __global__ void philox_kernel(const uint32_t* in, uint32_t* out, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -210,13 +211,14 @@ bool excitation_table(const int32_t* conf, int nconf, int n, std::vector<int32_t
 int refuse_excitations(const QmcPlan* plan) {
     if (plan && plan->t.exc_mode)
         return fail(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin: this entry point has no "
-                                         "excitation (trace-trick) path (psi, pdf, local energy and the Metropolis "
-                                         "sweep have)");
+                                         "excitation (trace-trick) path (psi, pdf, local energy, the Metropolis "
+                                         "sweep, the drift, HMC and the log|psi| adjoint have)");
     return QMC_OK;
 }
 
 int launch_hmc(const HArgs& a, int precision, hipStream_t s) {
     const bool f64 = precision == QMC_PREC_FP64, sto = a.t.radial == QMC_RADIAL_STO;
+    if (a.t.exc_mode) return QMC_BY_RAD(launch_hmc_exc, a.t.radial, a, precision, s);
     if (a.t.nconf_max <= 1) {
         if (f64) return sto ? launch_hmc_f64<QMC_RADIAL_STO, 1>(a, s) : launch_hmc_f64<QMC_RADIAL_GTO_PURE, 1>(a, s);
         return sto ? launch_hmc_f32<QMC_RADIAL_STO, 1>(a, s) : launch_hmc_f32<QMC_RADIAL_GTO_PURE, 1>(a, s);
@@ -317,6 +319,36 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
         t.ci_up = (decltype(t.ci_up))p->upload(d->ci_up, d->nci);
         t.ci_dn = (decltype(t.ci_dn))p->upload(d->ci_dn, d->nci);
         if (!t.conf_dn) t.conf_dn = t.conf_up;
+        {
+            // inverse maps for the adjoint slot sum: entries per AO, and the
+            // real exponent terms (a group's own primitives) per bas_exp index
+            std::vector<int32_t> ao_ptr(d->nao + 1, 0), ao_ent(d->nentry);
+            for (int i = 0; i < d->nentry; ++i) ++ao_ptr[d->gao_mu[i] + 1];
+            for (int mu = 0; mu < d->nao; ++mu) ao_ptr[mu + 1] += ao_ptr[mu];
+            {
+                std::vector<int32_t> fill(ao_ptr.begin(), ao_ptr.end() - 1);
+                for (int i = 0; i < d->nentry; ++i) ao_ent[fill[d->gao_mu[i]]++] = i;
+            }
+            std::vector<int32_t> exp_ptr(d->nprim_ref + 1, 0), exp_col;
+            std::vector<std::pair<int32_t, int32_t>> terms;  // (bas_exp index, column)
+            for (int g = 0; g < d->ngroup; ++g)
+                for (int i = d->grp_ao0[g]; i < d->grp_ao0[g] + d->grp_nao[g]; ++i)
+                    for (int q = 0; q < d->grp_nprim[g]; ++q)
+                        terms.emplace_back(d->gao_src[(size_t)i * QMC_MAXP + q], i * QMC_MAXP + q);
+            std::stable_sort(terms.begin(), terms.end(),
+                             [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+                                 return a.first < b.first;
+                             });
+            for (const auto& tc : terms) {
+                ++exp_ptr[tc.first + 1];
+                exp_col.push_back(tc.second);
+            }
+            for (int k = 0; k < d->nprim_ref; ++k) exp_ptr[k + 1] += exp_ptr[k];
+            t.ao_ptr = (decltype(t.ao_ptr))p->upload(ao_ptr.data(), ao_ptr.size());
+            t.ao_ent = (decltype(t.ao_ent))p->upload(ao_ent.data(), ao_ent.size());
+            t.exp_ptr = (decltype(t.exp_ptr))p->upload(exp_ptr.data(), exp_ptr.size());
+            t.exp_col = (decltype(t.exp_col))p->upload(exp_col.data(), exp_col.size());
+        }
         t.exc_mode = exc ? 1 : 0;
         if (exc) {
             t.exc_up = (decltype(t.exc_up))p->upload(exc_up.data(), exc_up.size());
@@ -435,7 +467,6 @@ int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precisio
                   const QmcHmcDesc* hm, qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
-    if ((rc = refuse_excitations(plan))) return rc;
     if (!wk || !hm || !wk->pos || !wk->logabs || !wk->sign) return fail(QMC_ERR_ARG, "null walker state");
     if (hm->traj_length < 1) return fail(QMC_ERR_ARG, "traj_length must be >= 1");
     if (hm->keep && hm->keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
@@ -455,7 +486,7 @@ int qmc_hmc_sweep(const QmcPlan* plan, const QmcParams* params, int32_t precisio
         return QMC_OK;
     }
     rc = launch_hmc(a, precision, s);
-    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, a.t.exc_mode ? kNoExcBucket : kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }
@@ -464,7 +495,6 @@ int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision
                  double* drift, double* logabs, double* sign, qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
-    if ((rc = refuse_excitations(plan))) return rc;
     if (!pos || !drift) return fail(QMC_ERR_ARG, "null pos/drift");
     if (!params->entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
     if (nw == 0) return QMC_OK;
@@ -483,7 +513,7 @@ int qmc_wf_drift(const QmcPlan* plan, const QmcParams* params, int32_t precision
         return QMC_OK;
     }
     rc = launch_hmc(a, precision, s);
-    if (rc) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    if (rc) return fail(QMC_ERR_UNSUPPORTED, a.t.exc_mode ? kNoExcBucket : kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }
@@ -550,7 +580,6 @@ int qmc_logpsi_grad(const QmcPlan* plan, const QmcParams* params, const double* 
                     qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
-    if ((rc = refuse_excitations(plan))) return rc;
     if (!out || !out->grad_w || !out->grad_ci || !out->grad_jas) return fail(QMC_ERR_ARG, "null gradient output");
     const RedSlots r = red_layout(plan->t, nw, out->grad_exp != nullptr, (double*)workspace, workspace_bytes,
                                   weights);
@@ -573,7 +602,9 @@ int qmc_logpsi_grad(const QmcPlan* plan, const QmcParams* params, const double* 
     a.pos = pos;
     a.nw = nw;
     a.r = r;
-    if (QMC_BY_RAD(launch_grad_reduce, a.t.radial, a, o, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    const int miss = a.t.exc_mode ? QMC_BY_RAD(launch_grad_reduce_exc, a.t.radial, a, o, s)
+                                  : QMC_BY_RAD(launch_grad_reduce, a.t.radial, a, o, s);
+    if (miss) return fail(QMC_ERR_UNSUPPORTED, a.t.exc_mode ? kNoExcBucket : kNoBucket);
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }

@@ -187,9 +187,10 @@ struct GRedArgs {
 // Adds g * d log|psi| / d theta of the wavefront's walkers to its slot
 // (grad_kernel's quantities: Gamma rows, D_n / S, the Pade weight
 // derivatives and the exponent terms, contracted instead of written).
-template <int NM, int RAD>
+// EXC: plans with exc_mode (Gamma from exc_gamma; xlds its LDS rows).
+template <int NM, int RAD, bool EXC = false>
 __device__ __forceinline__ void logpsi_adjoint_reduce(const GRedArgs& a, const WalkerLane& L, double* slot,
-                                                      double* stage) {
+                                                      double* stage, double* xlds = nullptr) {
     const Tables& t = a.t;
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
     const double x = p[0], y = p[1], z = p[2];
@@ -211,32 +212,50 @@ __device__ __forceinline__ void logpsi_adjoint_reduce(const GRedArgs& a, const W
         };
         ao_row<double, RAD, false>(t, a.p, x, y, z, sink);
     }
-    double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF], om[QMC_MAXCONF], gam[NM];
+    double gam[NM];
+    if constexpr (EXC) {
+        double lg, sg, S;
+        exc_gamma<NM>(t, a.p, L, mo, xlds, gam, lg, sg, S);
+        // d log|psi| / d c_n = R_n^up R_n^dn / S
+        const int blk = t.nsig_max * NM;
+        const double* Mu = xlds + (size_t)(L.wl * 2) * 3 * blk;
+        const double* Md = Mu + 3 * blk;
+        for (int n = 0; n < t.nci; ++n) {
+            double Ru, Rd = 1.0, q;
+            exc_ratio<NM, false>(t.exc_up + 6 * t.ci_up[n], Mu, nullptr, t.nup, 0.0, Ru, q);
+            if (t.ndown > 0) exc_ratio<NM, false>(t.exc_dn + 6 * t.ci_dn[n], Md, nullptr, t.ndown, 0.0, Rd, q);
+            const double sum = wave_sum((v && L.e == 0) ? gw * (Ru * Rd / S) : 0.0);
+            const int o = a.r.off_ci + n;
+            if (red_owner(o) == L.lane) slot[o] += sum;
+        }
+    } else {
+        double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF], om[QMC_MAXCONF];
 #pragma unroll
-    for (int c = 0; c < QMC_MAXCONF; ++c) {
-        ld[c] = 0.0;
-        sd[c] = 1.0;
-        td[c] = 0.0;
-        om[c] = 0.0;
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            ld[c] = 0.0;
+            sd[c] = 1.0;
+            td[c] = 0.0;
+            om[c] = 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < NM; ++m) gam[m] = 0.0;
+        conf_dets<NM, false>(t, L, mo, ld, sd, om, gam);
+        CiTerms ct;
+        ci_terms(t, a.p, L, ld, sd, td, ct);
+        for (int n = 0; n < t.nci; ++n) {
+            double l, sg, tr;
+            ct.term(t, n, l, sg, tr);
+            const double dn_s = sg * exp(l - ct.lmax) / ct.S;  // d log|psi| / d c_n
+            const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
+#pragma unroll
+            for (int c = 0; c < QMC_MAXCONF; ++c)
+                if (c == mine) om[c] += ldc(a.p.ci, n) * dn_s;
+            const double sum = wave_sum((v && L.e == 0) ? gw * dn_s : 0.0);
+            const int o = a.r.off_ci + n;
+            if (red_owner(o) == L.lane) slot[o] += sum;
+        }
+        conf_dets<NM, true>(t, L, mo, ld, sd, om, gam);
     }
-#pragma unroll
-    for (int m = 0; m < NM; ++m) gam[m] = 0.0;
-    conf_dets<NM, false>(t, L, mo, ld, sd, om, gam);
-    CiTerms ct;
-    ci_terms(t, a.p, L, ld, sd, td, ct);
-    for (int n = 0; n < t.nci; ++n) {
-        double l, sg, tr;
-        ct.term(t, n, l, sg, tr);
-        const double dn_s = sg * exp(l - ct.lmax) / ct.S;  // d log|psi| / d c_n
-        const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
-#pragma unroll
-        for (int c = 0; c < QMC_MAXCONF; ++c)
-            if (c == mine) om[c] += ldc(a.p.ci, n) * dn_s;
-        const double sum = wave_sum((v && L.e == 0) ? gw * dn_s : 0.0);
-        const int o = a.r.off_ci + n;
-        if (red_owner(o) == L.lane) slot[o] += sum;
-    }
-    conf_dets<NM, true>(t, L, mo, ld, sd, om, gam);
     // W: stage the recomputed AO row, contract with g * Gamma
     double gG[NM], unused[NM];
 #pragma unroll
@@ -335,6 +354,22 @@ __global__ __launch_bounds__(kBlock) void grad_reduce_kernel(GRedArgs a) {
     for (int64_t vw = slot_id; vw < a.r.nwaves; vw += a.r.nslot) {
         const WalkerLane L = lane_layout_w(a.t, a.nw, (int)vw);
         logpsi_adjoint_reduce<NM, RAD>(a, L, slot, stage);
+    }
+}
+
+// Plans with exc_mode: one wavefront per block (its exc_gamma rows in the
+// dynamic LDS, 2 (G + 1) groups of 3 x nsig_max x NM doubles), one slot each.
+template <int NM, int RAD>
+__global__ __launch_bounds__(64) void grad_reduce_kernel_exc(GRedArgs a) {
+    const int slot_id = blockIdx.x;
+    if (slot_id >= a.r.nslot) return;
+    const int lane = threadIdx.x & 63;
+    double* slot = a.r.ws + (size_t)slot_id * a.r.stride;
+    red_zero_slot(a.r, a.t.nentry, a.t.nmo, slot_id, lane);
+    __shared__ double stage[64 * (kRedCols + 1)];
+    for (int64_t vw = slot_id; vw < a.r.nwaves; vw += a.r.nslot) {
+        const WalkerLane L = lane_layout_w(a.t, a.nw, (int)vw);
+        logpsi_adjoint_reduce<NM, RAD, true>(a, L, slot, stage, (double*)qmc_dyn_lds);
     }
 }
 
@@ -543,9 +578,15 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_reduce_kernel(ERedArgs a) {
 }
 
 // ------------------------------------------------------------ slot reduction
-// Sums the slots: dW[mu][m] over the AO entries of mu, the other outputs
-// directly; exponent term slots (entry i, primitive q) go to bas_exp index
-// gao_src[i][q].  One thread per output.
+// Two launches.  slot_fold_kernel: slot p (p < nfold) receives, column by
+// column, the sum of the slots p, p + nfold, p + 2 nfold, ...  Block (x, p)
+// owns columns [64 x, 64 x + 64) of exactly those slots, so its in-place
+// write races with no other block; its four wavefronts take every fourth slot
+// of the partition and their sums are combined in a fixed order.
+// slot_reduce_kernel, one thread per output: dW[mu][m] over the AO entries of
+// mu, the other outputs directly, exponent terms (entry i, primitive q) to
+// bas_exp index gao_src[i][q] (the plan's inverse maps list both), each
+// summed over the nfold folded slots.
 struct RedOut {
     double* grad_w;    // [nao][nmo]
     double* grad_ci;   // [nci]
@@ -553,40 +594,75 @@ struct RedOut {
     double* grad_exp;  // [nprim_ref] or null (its exponent terms must be in the slots)
 };
 
-static __global__ __launch_bounds__(kBlock) void slot_reduce_kernel(Tables t, RedSlots r, RedOut o) {
+constexpr int kFoldBlock = 256;  // four wavefronts per block of slot_fold_kernel
+constexpr int kMaxFold = 64;     // folded slots left for slot_reduce_kernel
+
+inline int red_nfold(const RedSlots& r) { return r.nslot < kMaxFold ? r.nslot : kMaxFold; }
+
+static __global__ __launch_bounds__(kFoldBlock) void slot_fold_kernel(RedSlots r, int nfold) {
+    __shared__ double part[kFoldBlock / 64][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    const int64_t p = blockIdx.y;
+    double acc = 0.0;
+    if (col < r.stride) {
+#pragma unroll 4
+        for (int64_t sl = p + (int64_t)nfold * w; sl < r.nslot; sl += (int64_t)nfold * (kFoldBlock / 64))
+            acc += r.ws[sl * r.stride + col];
+    }
+    part[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && col < r.stride) {
+        double sum = part[0][lane];
+#pragma unroll
+        for (int k = 1; k < kFoldBlock / 64; ++k) sum += part[k][lane];
+        r.ws[p * r.stride + col] = sum;
+    }
+}
+
+static __global__ __launch_bounds__(kBlock) void slot_reduce_kernel(Tables t, RedSlots r, RedOut o, int nfold) {
     const int n_w = t.nao * t.nmo;
     const int n_all = n_w + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n_all; idx += gridDim.x * blockDim.x) {
         double acc = 0.0;
         if (idx < n_w) {
             const int mu = idx / t.nmo, m = idx - mu * t.nmo;
-            for (int i = 0; i < t.nentry; ++i) {
-                if (t.gao_mu[i] != mu) continue;
-                for (int s = 0; s < r.nslot; ++s) acc += r.ws[(size_t)s * r.stride + i * t.nmo + m];
+            const int e0 = t.ao_ptr[mu], e1 = t.ao_ptr[mu + 1];
+            for (int e = e0; e < e1; ++e) {
+                const int col = t.ao_ent[e] * t.nmo + m;
+                for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + col];
             }
             o.grad_w[idx] = acc;
         } else if (idx < n_w + t.nci) {
             const int n = idx - n_w;
-            for (int s = 0; s < r.nslot; ++s) acc += r.ws[(size_t)s * r.stride + r.off_ci + n];
+            for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + r.off_ci + n];
             o.grad_ci[n] = acc;
         } else if (idx < n_w + t.nci + 2) {
             const int k = idx - n_w - t.nci;
-            for (int s = 0; s < r.nslot; ++s) acc += r.ws[(size_t)s * r.stride + r.off_jas + k];
+            for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + r.off_jas + k];
             o.grad_jas[k] = acc;
         } else {
             const int src = idx - n_w - t.nci - 2;  // bas_exp index
-            for (int i = 0; i < t.nentry; ++i)
-                for (int q = 0; q < QMC_MAXP; ++q) {
-                    if (t.gao_src[(size_t)i * QMC_MAXP + q] != src) continue;
-                    // only the group's primitives are real terms (padding slots repeat index 0)
-                    int g = 0;
-                    while (g + 1 < t.ngroup && t.grp_ao0[g + 1] <= i) ++g;
-                    if (q >= t.grp_nprim[g]) continue;
-                    for (int s = 0; s < r.nslot; ++s) acc += r.ws[(size_t)s * r.stride + r.off_exp + i * QMC_MAXP + q];
-                }
+            const int c0 = t.exp_ptr[src], c1 = t.exp_ptr[src + 1];
+            for (int c = c0; c < c1; ++c) {
+                const int col = r.off_exp + t.exp_col[c];
+                for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + col];
+            }
             o.grad_exp[src] = acc;
         }
     }
+}
+
+// The slot sum on stream s: the fold (when there are more slots than
+// kMaxFold) and the per-output reduction.
+inline void red_slot_sum(const Tables& t, const RedSlots& r, const RedOut& o, hipStream_t s) {
+    const int nfold = red_nfold(r);
+    if (r.nslot > nfold)
+        hipLaunchKernelGGL(slot_fold_kernel, dim3((unsigned)((r.stride + 63) / 64), (unsigned)nfold), dim3(kFoldBlock),
+                           0, s, r, nfold);
+    const int n = t.nao * t.nmo + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
+    hipLaunchKernelGGL(slot_reduce_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t, r, o,
+                       nfold);
 }
 
 // Layout of a slot and the slot count for nw walkers within ws_bytes.

@@ -56,6 +56,14 @@ struct Tables {
     const QMC_CONST int* ci_dn;
     const QMC_CONST int* exc_up;  // [nconf_up][6] {level, h1, h2, p1, p2, sign}, or null
     const QMC_CONST int* exc_dn;
+    // inverse maps of the adjoint slot sum (CSR): the AO entries of AO mu are
+    // ao_ent[ao_ptr[mu] .. ao_ptr[mu + 1]), the exponent-term columns
+    // (entry i, primitive q) -> i * QMC_MAXP + q of bas_exp index k are
+    // exp_col[exp_ptr[k] .. exp_ptr[k + 1]), both ascending
+    const QMC_CONST int* ao_ptr;
+    const QMC_CONST int* ao_ent;
+    const QMC_CONST int* exp_ptr;
+    const QMC_CONST int* exp_col;
 };
 
 // ------------------------------------------------------------------ math

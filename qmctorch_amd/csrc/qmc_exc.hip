@@ -1,6 +1,7 @@
 // Excitation (trace-trick) kernels for plans with more than QMC_MAXCONF unique
 // configurations per spin: psi / local energy and the Metropolis sweep, both
 // precisions, register buckets up to 16.
+#include "qmc_adjoint.h"
 #include "qmc_launch.h"
 
 #ifndef QMC_RAD
@@ -8,17 +9,6 @@
 #endif
 
 namespace qmc {
-
-// dynamic LDS of one wavefront: 2 (G + 1) groups of M, X and Mk rows
-static size_t exc_lds_bytes(const Tables& t, int nm) {
-    const int G = 64 / t.nelec;
-    return (size_t)2 * (G + 1) * 3 * t.nsig_max * nm * sizeof(double);
-}
-
-static dim3 exc_grid(const Tables& t, int64_t nw) {
-    const int G = 64 / t.nelec;
-    return dim3((unsigned)((nw + G - 1) / G));
-}
 
 template <int NM, int RAD>
 static void go_wf(const KArgs& a, int prec, int mode, hipStream_t s) {
@@ -47,8 +37,6 @@ static void go_sweep(const SArgs& a, int prec, hipStream_t s) {
         hipLaunchKernelGGL((sweep_kernel_exc<float, NM, RAD>), g, dim3(64), lds, s, a);
 }
 
-#define QMC_EXC_BUCKETS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(12) X(16)
-
 template <>
 int launch_wf_exc<QMC_RAD>(const KArgs& a, int prec, int mode, hipStream_t s) {
     switch (nm_bucket_fine(a.t)) {
@@ -73,6 +61,27 @@ int launch_sweep_exc<QMC_RAD>(const SArgs& a, int prec, hipStream_t s) {
 #undef QMC_CASE
         default: return -1;
     }
+}
+
+template <int NM, int RAD>
+static void go_grad_reduce(const GRedArgs& a, hipStream_t s) {
+    const size_t lds = exc_lds_bytes(a.t, NM);
+    hipLaunchKernelGGL((grad_reduce_kernel_exc<NM, RAD>), dim3((unsigned)a.r.nslot), dim3(64), lds, s, a);
+}
+
+template <>
+int launch_grad_reduce_exc<QMC_RAD>(const GRedArgs& a, const RedOut& o, hipStream_t s) {
+    switch (nm_bucket_fine(a.t)) {
+#define QMC_CASE(K)                          \
+    case K:                                  \
+        go_grad_reduce<K, QMC_RAD>(a, s);    \
+        break;
+        QMC_EXC_BUCKETS(QMC_CASE)
+#undef QMC_CASE
+        default: return -1;
+    }
+    red_slot_sum(a.t, a.r, o, s);
+    return 0;
 }
 
 }  // namespace qmc

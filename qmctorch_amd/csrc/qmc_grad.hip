@@ -62,11 +62,6 @@ int launch_eloc_grad<QMC_RAD>(const EArgs& a, hipStream_t s) {
     return bucket_e<QMC_RAD>(a, s);
 }
 
-static void slot_sum(const Tables& t, const RedSlots& r, const RedOut& o, hipStream_t s) {
-    const int n = t.nao * t.nmo + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
-    hipLaunchKernelGGL(slot_reduce_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t, r, o);
-}
-
 template <>
 int launch_grad_reduce<QMC_RAD>(const GRedArgs& a, const RedOut& o, hipStream_t s) {
     const dim3 grid = grid_for_waves(a.r.nslot);
@@ -78,7 +73,7 @@ int launch_grad_reduce<QMC_RAD>(const GRedArgs& a, const RedOut& o, hipStream_t 
         case 24: hipLaunchKernelGGL((grad_reduce_kernel<24, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
         default: return -1;
     }
-    slot_sum(a.t, a.r, o, s);
+    red_slot_sum(a.t, a.r, o, s);
     return 0;
 }
 
@@ -93,7 +88,7 @@ int launch_eloc_reduce<QMC_RAD>(const ERedArgs& a, const RedOut& o, hipStream_t 
         case 24: hipLaunchKernelGGL((eloc_grad_reduce_kernel<24, QMC_RAD>), grid, dim3(kBlock), 0, s, a); break;
         default: return -1;
     }
-    slot_sum(a.t, a.r, o, s);
+    red_slot_sum(a.t, a.r, o, s);
     return 0;
 }
 

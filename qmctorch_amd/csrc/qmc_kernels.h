@@ -646,6 +646,141 @@ __device__ __forceinline__ void slater_exc(const Tables& t, const QmcParams& prm
     kin = KIN ? ST / S : 0.0;
 }
 
+// Gamma row of this lane's electron i for plans with exc_mode (the adjoint of
+// slater_exc): Gamma[j] = d log|psi_S| / d Phi[i][j] for the MO columns
+// j < nmo, Phi the lane's MO row.  With A the ground-state block, M = A^-1 V,
+// R_n^s the ratio of CI term n for spin s and S = sum_n c_n R_n^up R_n^dn,
+//   d log|psi_S| = tr(A^-1 dA) + (1/S) sum_n c_n (dR_n^s R_n^s' + ...),
+//   dR_n^s = sum_{h,p} (dR_n^s / dM[h][p]) dM[h][p],   dM = A^-1 (dV - dA M),
+// so with G[h][p] = (1/S) sum_n c_n R_n^s' dR_n^s / dM[h][p] (s' the other
+// spin) and Z = [I - G M^T | G] (n x nmo),
+//   Gamma[i][:] = sum_h (A^-1)[h][i] Z[h][:].
+// dR/dM: single (h, p): sign; double (h1 h2, p1 p2), R = sign det T:
+// dR/dT11 = sign T22, dR/dT12 = -sign T21, dR/dT21 = -sign T12, dR/dT22 =
+// sign T11.  The lane that pivoted column h of the eliminations of [A | Phi]
+// and [A | I] stores row h of M and of A^-1 in its group's LDS block and
+// computes row h of G and Z.  lds: 2 (G + 1) groups of M, A^-1 and Z rows
+// (nsig_max x NM doubles each), group (walker wl, spin).  Also returns log|S
+// D_0^up D_0^dn|, its sign and S.
+template <int NM>
+__device__ __forceinline__ void exc_gamma(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                          const double (&mo)[NM], double* lds, double (&gam)[NM], double& logabs,
+                                          double& sgn, double& S) {
+    const int n = L.n_sig;
+    const int blk = t.nsig_max * NM;
+    double* Mb = lds + (size_t)(L.wl * 2 + L.spin) * 3 * blk;
+    double* Ab = Mb + blk;
+    double* Zb = Ab + blk;
+    double a[NM], b[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? mo[j] : 0.0;
+        b[j] = mo[j];
+    }
+    double lg, sg, tp;
+    int col;
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    wave_lds_sync();  // the previous walkers' reads are done
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Mb[col * NM + j] = b[j];  // row col of [I | M]
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? mo[j] : 0.0;
+        b[j] = (j == L.myrow) ? 1.0 : 0.0;
+    }
+    double lg2, sg2;
+    int col2;
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg2, sg2, tp, col2);
+    if (L.lane_real && col2 >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Ab[col2 * NM + j] = b[j];  // row col2 of A^-1
+    }
+    wave_lds_sync();
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    const double lu = __shfl(lg, up_lane), su = __shfl(sg, up_lane);
+    const double ld = __shfl(lg, dn_lane), sd = __shfl(sg, dn_lane);
+    const double* Mu = lds + (size_t)(L.wl * 2) * 3 * blk;
+    const double* Md = Mu + 3 * blk;
+    double s = 0.0;
+    for (int m = L.e; m < t.nci; m += t.nelec) {
+        double Ru, Rd = 1.0, q;
+        exc_ratio<NM, false>(t.exc_up + 6 * t.ci_up[m], Mu, nullptr, t.nup, 0.0, Ru, q);
+        if (t.ndown > 0) exc_ratio<NM, false>(t.exc_dn + 6 * t.ci_dn[m], Md, nullptr, t.ndown, 0.0, Rd, q);
+        s += ldc(prm.ci, m) * Ru * Rd;
+    }
+    S = seg_sum(s, L.wbase, t.nelec, t.nelec);
+    logabs = lu + (t.ndown > 0 ? ld : 0.0) + log(fabs(S));
+    sgn = su * (t.ndown > 0 ? sd : 1.0) * (S < 0.0 ? -1.0 : 1.0);
+    // row h = col of G for this lane's spin
+    const bool up = L.spin == 0;
+    const QMC_CONST int* ex_me = up ? t.exc_up : t.exc_dn;
+    const QMC_CONST int* ex_ot = up ? t.exc_dn : t.exc_up;
+    const double* Mo = up ? Md : Mu;
+    const int n_ot = up ? t.ndown : t.nup;
+    const bool other = up ? t.ndown > 0 : true;
+    double g[NM];
+#pragma unroll
+    for (int p = 0; p < NM; ++p) g[p] = 0.0;
+    auto add = [&](int p, double v) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q)
+            if (q == p) g[q] += v;
+    };
+    for (int m = 0; m < t.nci; ++m) {
+        const QMC_CONST int* e = ex_me + 6 * (up ? t.ci_up[m] : t.ci_dn[m]);
+        const int lvl = e[0];
+        if (lvl == 0) continue;
+        const int h1 = e[1], h2 = e[2], p1 = e[3], p2 = e[4];
+        if (h1 != col && !(lvl == 2 && h2 == col)) continue;
+        double Ro = 1.0, q;
+        if (other) exc_ratio<NM, false>(ex_ot + 6 * (up ? t.ci_dn[m] : t.ci_up[m]), Mo, nullptr, n_ot, 0.0, Ro, q);
+        const double cf = ldc(prm.ci, m) * Ro * (double)e[5] / S;
+        if (lvl == 1) {
+            add(p1, cf);
+        } else {
+            const double t11 = Mb[h1 * NM + n + p1], t12 = Mb[h1 * NM + n + p2];
+            const double t21 = Mb[h2 * NM + n + p1], t22 = Mb[h2 * NM + n + p2];
+            if (h1 == col) {
+                add(p1, cf * t22);
+                add(p2, -cf * t21);
+            }
+            if (h2 == col) {
+                add(p2, cf * t11);
+                add(p1, -cf * t12);
+            }
+        }
+    }
+    // row h = col of Z = [I - G M^T | G]
+    double z[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) z[j] = (j == col) ? 1.0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < NM; ++p) {
+        if (n + p < t.nmo) {
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                if (j < n) z[j] -= g[p] * Mb[j * NM + n + p];
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                if (j == n + p) z[j] = g[p];
+        }
+    }
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Zb[col * NM + j] = z[j];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < NM; ++j) gam[j] = 0.0;
+    for (int h = 0; h < n; ++h) {
+        const double y = Ab[h * NM + L.myrow];  // (A^-1)[h][i]
+#pragma unroll
+        for (int j = 0; j < NM; ++j) gam[j] += y * Zb[h * NM + j];
+    }
+}
+
 struct KArgs {
     Tables t;
     QmcParams p;
@@ -1174,6 +1309,41 @@ __device__ __forceinline__ double eval_drift(const Tables& t, const QmcParams& p
     return logabs + logj;
 }
 
+// log|psi| (every lane) and the drift of this lane's electron for plans with
+// exc_mode: grad_i ln|psi_S| = sum_j Gamma[i][j] grad phi_j(r_i) with the
+// Gamma row of exc_gamma (FP64, from the MO row in T), plus grad_i ln J.
+// xlds: the wavefront's exc_gamma rows (dynamic LDS).
+template <typename T, int NM, int RAD>
+__device__ __forceinline__ double eval_drift_exc(const Tables& t, const QmcParams& prm, const WalkerLane& L, double x,
+                                                 double y, double z, double* xlds, double& sgn, double (&dr)[3]) {
+    T mo[NM], gx[NM], gy[NM], gz[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) mo[m] = gx[m] = gy[m] = gz[m] = T(0);
+    MoSink<T, NM, true, T, false> sink{(const T*)prm.entry_w, t.nmo, mo, gx, gy, gz, nullptr};
+    AoGeneric::template row<T, RAD, true, false>(t, prm, x, y, z, sink);
+    T jg[3];
+    const T kj = jastrow_value_grad<T>(t, prm, L, x, y, z, jg);
+    const double logj = (double)seg_sum<T>(kj, L.wbase, t.nelec, t.nelec);
+    double m64[NM], gam[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) m64[m] = (double)mo[m];
+    double logabs, S;
+    exc_gamma<NM>(t, prm, L, m64, xlds, gam, logabs, sgn, S);
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        if (m < t.nmo) {
+            d0 += gam[m] * (double)gx[m];
+            d1 += gam[m] * (double)gy[m];
+            d2 += gam[m] * (double)gz[m];
+        }
+    }
+    dr[0] = d0 + (double)jg[0];
+    dr[1] = d1 + (double)jg[1];
+    dr[2] = d2 + (double)jg[2];
+    return logabs + logj;
+}
+
 // Hamiltonian Monte Carlo steps with the walker state, momenta and the drift
 // at the current position in registers (sampler/hamiltonian.py:80-201).  With
 // nstep == 0 the kernel only evaluates log|psi|, sign and the drift at the
@@ -1188,11 +1358,17 @@ struct HArgs {
     int write_pos;
 };
 
-template <typename T, int NM, int RAD, int MC, typename AO = AoGeneric>
+// EXC: plans with exc_mode, launched one wavefront per block with the
+// exc_gamma rows in the dynamic LDS.
+template <typename T, int NM, int RAD, int MC, typename AO = AoGeneric, bool EXC = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 8 ? 2 : 1))) void hmc_kernel(HArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
-    double* lds = wave_rowbuf<NM>();  // row transposes; above 8 rows also the pivot rows
+    double* lds;  // row transposes (above 8 rows also the pivot rows), or the exc_gamma rows
+    if constexpr (EXC)
+        lds = (double*)qmc_dyn_lds;
+    else
+        lds = wave_rowbuf<NM>();
     const Tables& t = a.t;
     const int Ne = t.nelec;
     double* pp = a.wk.pos + (size_t)L.wclamp * 3 * Ne + 3 * L.e;
@@ -1212,7 +1388,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM <= 8 
     int s = -1, l = 0;
     for (;;) {
         double sgn_n, gn[3];
-        const double lpn = eval_drift<T, NM, RAD, MC, AO>(t, a.p, L, qx, qy, qz, logc0, lds, sgn_n, gn);
+        double lpn;
+        if constexpr (EXC)
+            lpn = eval_drift_exc<T, NM, RAD>(t, a.p, L, qx, qy, qz, lds, sgn_n, gn);
+        else
+            lpn = eval_drift<T, NM, RAD, MC, AO>(t, a.p, L, qx, qy, qz, logc0, lds, sgn_n, gn);
         if (s >= 0) {
             // momentum update at the new point: full step, or the closing half step
             const double f = (l + 1 < traj) ? 2.0 * eps : eps;  // p -= f/2 grad U, grad U = -2 drift

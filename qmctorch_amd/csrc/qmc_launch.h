@@ -53,6 +53,10 @@ struct ERedArgs;
 struct RedOut;
 template <int RAD> int launch_grad_reduce(const GRedArgs& a, const RedOut& o, hipStream_t s);
 template <int RAD> int launch_eloc_reduce(const ERedArgs& a, const RedOut& o, hipStream_t s);
+// the log|psi| adjoint, the drift and HMC of plans with exc_mode (register
+// buckets up to 16)
+template <int RAD> int launch_grad_reduce_exc(const GRedArgs& a, const RedOut& o, hipStream_t s);
+template <int RAD> int launch_hmc_exc(const HArgs& a, int precision, hipStream_t s);
 // excitation (trace-trick) kernels, any precision; register buckets up to 16
 template <int RAD> int launch_wf_exc(const KArgs& a, int precision, int mode, hipStream_t s);
 template <int RAD> int launch_sweep_exc(const SArgs& a, int precision, hipStream_t s);
@@ -68,6 +72,18 @@ template <int RAD>
 int launch_wf_batch_f64(const WJob* jobs, int njobs, int nwaves, int bucket, int mode, hipStream_t s);
 
 inline dim3 grid_for_waves(int nwaves) { return dim3((unsigned)((nwaves + (kBlock / 64) - 1) / (kBlock / 64))); }
+
+// Excitation-path launches: one wavefront per block, its dynamic LDS holding
+// 2 (G + 1) groups (walker, spin) of three nsig_max x nm row blocks.
+inline size_t exc_lds_bytes(const Tables& t, int nm) {
+    const int G = 64 / t.nelec;
+    return (size_t)2 * (G + 1) * 3 * t.nsig_max * nm * sizeof(double);
+}
+inline dim3 exc_grid(const Tables& t, int64_t nw) {
+    const int G = 64 / t.nelec;
+    return dim3((unsigned)((nw + G - 1) / G));
+}
+#define QMC_EXC_BUCKETS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(12) X(16)
 
 // Kernels built for one basis layout (generated units, specialize.py).  Each
 // launcher returns 0, or -1 when the layout has no instance for that

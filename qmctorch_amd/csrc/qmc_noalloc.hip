@@ -58,8 +58,6 @@ int qmc_logpsi_grad_noalloc(const QmcPlan* plan, const QmcParams* params, const 
         return set_error(QMC_ERR_ARG, "missing parameter pointer");
     if (plan->t.use_ee && !params->w_ee) return set_error(QMC_ERR_ARG, "e-e Jastrow weight missing");
     if (plan->t.use_en && !params->w_en) return set_error(QMC_ERR_ARG, "e-n Jastrow weight missing");
-    if (plan->t.exc_mode)
-        return set_error(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin: no adjoint path");
     if (!out || !out->grad_w || !out->grad_ci || !out->grad_jas)
         return set_error(QMC_ERR_ARG, "null gradient output");
     if (nw == 0) return qmc_logpsi_grad(plan, params, pos, 0, weights, out, nullptr, 0, stream);
@@ -80,8 +78,11 @@ int qmc_logpsi_grad_noalloc(const QmcPlan* plan, const QmcParams* params, const 
     a.r = r;
     RedOut o{out->grad_w, out->grad_ci, out->grad_jas, out->grad_exp};
     hipStream_t s = (hipStream_t)stream;
-    if (QMC_BY_RAD(launch_grad_reduce, a.t.radial, a, o, s))
-        return set_error(QMC_ERR_UNSUPPORTED, "more than 24 occupied orbitals per spin");
+    const int miss = a.t.exc_mode ? QMC_BY_RAD(launch_grad_reduce_exc, a.t.radial, a, o, s)
+                                  : QMC_BY_RAD(launch_grad_reduce, a.t.radial, a, o, s);
+    if (miss)
+        return set_error(QMC_ERR_UNSUPPORTED, a.t.exc_mode ? "more than 16 MO columns per electron (excitation path)"
+                                                           : "more than 24 occupied orbitals per spin");
     QMC_CHECK_HIP(hipGetLastError());
     return QMC_OK;
 }

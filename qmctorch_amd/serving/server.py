@@ -102,6 +102,21 @@ def _stats(e: np.ndarray):
     return mean, var, float(np.sqrt(var / n)) if n > 0 else float("nan")
 
 
+def _tick_summary(ticks) -> Dict[str, float]:
+    """Percentiles of the per-tick host submission time, device time and the
+    device idle gap before each tick (every event has completed)."""
+    host = np.array([h for h, _, _, _ in ticks]) * 1e6
+    dev = np.array([a.elapsed_time(b) for _, a, b, _ in ticks]) * 1e3
+    gap = np.array([ticks[k - 1][2].elapsed_time(ticks[k][1]) for k in range(1, len(ticks))]) * 1e3
+    gap = np.maximum(gap, 0.0)
+    pct = lambda x, q: float(np.percentile(x, q)) if len(x) else 0.0
+    return {"ticks": len(ticks), "jobs_per_tick_mean": float(np.mean([n for _, _, _, n in ticks])),
+            "host_submit_us_p50": pct(host, 50), "host_submit_us_p99": pct(host, 99),
+            "device_tick_us_p50": pct(dev, 50), "device_tick_us_p99": pct(dev, 99),
+            "idle_gap_us_p50": pct(gap, 50), "idle_gap_us_p99": pct(gap, 99),
+            "device_busy_s": float(dev.sum()) * 1e-6, "host_submit_s": float(host.sum()) * 1e-6}
+
+
 class VMCServer:
     """Continuous-batching request loop (one GPU, one host thread)."""
 
@@ -136,10 +151,16 @@ class VMCServer:
         return req.nwalkers * self.context(req.system).walker_bytes()
 
     def run(self, requests: Sequence[VMCRequest], realtime: bool = True,
-            clock: Callable[[], float] = time.perf_counter) -> List[VMCResult]:
+            clock: Callable[[], float] = time.perf_counter, profile_ticks: bool = False) -> List[VMCResult]:
         """Serve ``requests`` (arrival times relative to the start) and return
         their results in completion order.  With realtime=False every request
-        is available from the start (arrival times only order the queue)."""
+        is available from the start (arrival times only order the queue).
+
+        profile_ticks: time every tick (timing events around its launches,
+        the host clock around its submission) and leave the summary in
+        ``self.tick_profile``: host submission time per tick (what a captured
+        graph could shorten), device time per tick, and the device idle gap
+        between consecutive ticks (zero when the host keeps ahead)."""
         for r in requests:
             if self.request_bytes(r) > self.hbm_budget:
                 raise ValueError("request %d needs more device memory than the whole budget" % r.rid)
@@ -151,6 +172,7 @@ class VMCServer:
         used = 0
         self.peak_bytes = self.peak_active = 0
         results: List[VMCResult] = []
+        ticks = [] if profile_ticks else None
         t0 = clock()
 
         def now() -> float:
@@ -190,6 +212,10 @@ class VMCServer:
             if active:
                 self.peak_active = max(self.peak_active, len(active))
                 jobs = [(a.walkers, min(a.remaining, self.steps_per_tick)) for a in active]
+                if ticks is not None:
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev0.record()
+                    h0 = time.perf_counter()
                 self.runner.sweep(jobs)
                 fin = []
                 keep = []
@@ -205,13 +231,11 @@ class VMCServer:
                         a.eloc_host.copy_(a.eloc_dev, non_blocking=True)
                         a.nacc_host = torch.empty(a.req.nwalkers, dtype=torch.int32, pin_memory=True)
                         a.nacc_host.copy_(a.walkers.naccept, non_blocking=True)
-                    ev = torch.cuda.Event()
-                    ev.record()
-                    inflight.append((ev, fin))
-                else:
-                    ev = torch.cuda.Event()
-                    ev.record()
-                    inflight.append((ev, []))
+                ev = torch.cuda.Event(enable_timing=ticks is not None)
+                ev.record()
+                inflight.append((ev, fin))
+                if ticks is not None:
+                    ticks.append((time.perf_counter() - h0, ev0, ev, len(jobs)))
                 active = keep
                 self.ticks += 1
                 drain(block=len(inflight) > self.lookahead)
@@ -224,6 +248,8 @@ class VMCServer:
                     time.sleep(min(dt, 1e-3))
             elif queue:
                 raise RuntimeError("queued request cannot be admitted (budget or capacity too small)")
+        if ticks is not None:
+            self.tick_profile = _tick_summary(ticks)
         return results
 
 

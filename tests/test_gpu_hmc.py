@@ -64,8 +64,11 @@ def _host_noise(seed, nw, nelec, step):
     return torch.as_tensor(p), torch.as_tensor(u)
 
 
-@pytest.mark.parametrize("case", ["h2_sto3g_ee", "h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22"])
+@pytest.mark.parametrize("case", ["h2_sto3g_ee", "h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22",
+                                  "lih_631g_sd46_ee_en", "h2o_ccpvdz_sd46_ee"])
 def test_drift_fp64_vs_oracle(case):
+    """FP64 drift = Jacobi gradient / psi of the oracle (sd46: 15 unique
+    configurations per spin, the excitation path's Gamma rows)."""
     _say("case drift fp64", case)
     mol, kw = helpers.CASES[case]()
     wf, orc = helpers.make_pair(mol, **kw)
@@ -79,9 +82,10 @@ def test_drift_fp64_vs_oracle(case):
     assert torch.equal(sg.cpu(), torch.sign(psi))
 
 
-def test_drift_mixed_vs_fp64():
-    _say("case drift mixed")
-    mol, kw = helpers.CASES["h2o_ccpvdz_ee_en"]()
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "h2o_ccpvdz_sd46_ee"])
+def test_drift_mixed_vs_fp64(case):
+    _say("case drift mixed", case)
+    mol, kw = helpers.CASES[case]()
     wf, _ = helpers.make_pair(mol, precision="mixed", **kw)
     pos = helpers.positions(mol, 512, seed=22).to(DEV)
     d32, la32, _ = wf.drift(pos, "mixed")
@@ -94,7 +98,7 @@ def test_drift_mixed_vs_fp64():
 
 
 @pytest.mark.parametrize("case,eps", [("h2o_ccpvdz_ee_en", 0.1), ("lih_631g_cas22_ee_en", 0.1),
-                                      ("lih_631g_cas22_ee_en", 0.8)])
+                                      ("lih_631g_cas22_ee_en", 0.8), ("lih_631g_sd46_ee_en", 0.1)])
 def test_hmc_fp64_replays_oracle_step(case, eps):
     """Same momenta and uniforms: the device chain is the oracle's chain."""
     _say("case hmc replay", case, eps)

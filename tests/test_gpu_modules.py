@@ -179,7 +179,7 @@ def test_potentials_and_nuclear_repulsion():
     assert float((el1 - el0.cpu()).abs().max()) > 1e-3
 
 
-@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en"])
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_631g_sd46_ee_en"])
 def test_position_gradient_of_psi(case):
     """wf(x).backward() gives d psi / d x (through the drift kernel) equal to
     autograd through the oracle."""

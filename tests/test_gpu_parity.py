@@ -425,13 +425,13 @@ def test_solver_run_auto_gradient():
 
 def test_excitation_plans_refuse_entry_points_without_that_path():
     """Plans with more than 8 unique configurations per spin run psi, pdf, the
-    local energy and the sweep through the excitation kernels; the adjoints,
-    the drift and the batched launches say that they have no such path."""
+    local energy, the sweep, the drift / HMC and the log|psi| adjoint through
+    the excitation kernels; the local-energy adjoint (grad="auto") says that it
+    has no such path."""
     _say("case excitation refusals")
     mol, kw = helpers.CASES["lih_631g_sd46_ee_en"]()
     wf, _ = helpers.make_pair(mol, **kw)
     pos = helpers.positions(mol, 8, seed=2).to(DEV)
-    for call in (lambda: wf.gradients_jacobi(pos), lambda: wf.drift(pos),
-                 lambda: (wf(pos).sum()).backward()):
-        with pytest.raises(RuntimeError, match="excitation"):
-            call()
+    wf.ao.bas_exp.requires_grad_(False)
+    with pytest.raises(RuntimeError, match="excitation"):
+        wf.local_energy_autodiff(pos).sum().backward()

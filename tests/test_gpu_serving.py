@@ -186,12 +186,17 @@ def test_server_realtime_poisson(contexts):
     import time
 
     t0 = time.perf_counter()
-    res = server.run(reqs, realtime=True)
+    res = server.run(reqs, realtime=True, profile_ticks=True)
     wall = time.perf_counter() - t0
     s = latency_summary(res, wall)
     _say("  ", s)
     assert s["requests"] == 40 and all(r.done >= r.arrival for r in res)
     assert all(np.isfinite(r.energy) for r in res)
+    tp = server.tick_profile
+    _say("   ticks", tp)
+    assert tp["ticks"] > 0
+    assert tp["device_tick_us_p50"] > 0.0 and tp["host_submit_us_p50"] > 0.0
+    assert 0.0 < tp["device_busy_s"] < wall
 
 
 def test_config4_sixty_four_uneven_systems(contexts):

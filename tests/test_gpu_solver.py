@@ -94,10 +94,14 @@ def _fixed_batch(mol, n, seed):
     return helpers.positions(mol, n, seed=seed)
 
 
-@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22"])
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22",
+                                  "lih_631g_sd46_ee_en", "h2o_ccpvdz_sd46_ee"])
 def test_solver_manual_gradient_matches_oracle(case):
     """The gradients that Solver.evaluate_grad_manual leaves in .grad equal
-    O.manual_gradient (itself pinned by finite differences) on fixed positions."""
+    O.manual_gradient (itself pinned by finite differences) on fixed positions.
+    The sd46 cases (single_double(4,6): 15 unique configurations per spin, 93
+    CI terms) take the excitation (trace-trick) adjoint; the oracle takes
+    every determinant explicitly."""
     _say("case solver manual gradient", case)
     mol, kw = helpers.CASES[case]()
     wf, orc = helpers.make_pair(mol, **kw)
@@ -213,11 +217,13 @@ def test_backward_of_empty_batch_gives_zero_gradients():
     assert float(wf.mo.mo_modifier.grad.abs().max()) == 0.0
 
 
-def test_solver_run_epoch_matches_oracle_epoch():
+@pytest.mark.parametrize("case", ["lih_631g_cas22_ee_en", "lih_631g_sd46_ee_en"])
+def test_solver_run_epoch_matches_oracle_epoch(case):
     """One epoch of Solver.run's manual gradient + SGD step on fixed positions
-    (resampling off) moves every parameter as the same step on the oracle."""
-    _say("case solver epoch")
-    mol, kw = helpers.CASES["lih_631g_cas22_ee_en"]()
+    (resampling off) moves every parameter as the same step on the oracle
+    (sd46: the excitation-plan adjoint)."""
+    _say("case solver epoch", case)
+    mol, kw = helpers.CASES[case]()
     wf, orc = helpers.make_pair(mol, **kw)
     solver = _solver(wf, mol, 400, 1)
     solver.configure(grad="manual", freeze=["ao"], track=["local_energy"],
