@@ -295,7 +295,7 @@ def batched_uneven(nsys=64, steps=100, seed=0, precision="mixed"):
             "speedup": res["batched"] / res["per_system"]}
 
 
-def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0):
+def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0, lookahead=1):
     """Poisson request streams (config-4 molecule set, walkers 128-16384,
     1-20 sweeps + one E_L batch): continuous batching vs the plain
     per-request path on the same GPU."""
@@ -303,7 +303,7 @@ def serving(rates, nreq, precision="mixed", steps_per_tick=4, plain=True, seed=0
                                       serve_per_request)
 
     ctx = {n: SystemContext(catalog.wavefunction(n, precision), n) for n in catalog.SYSTEM_NAMES}
-    server = VMCServer(precision=precision, steps_per_tick=steps_per_tick, contexts=ctx)
+    server = VMCServer(precision=precision, steps_per_tick=steps_per_tick, contexts=ctx, lookahead=lookahead)
     server.run(poisson_requests(1000.0, 10, seed=99), realtime=False)  # warm every kernel
     out = []
     for rate in rates:
@@ -469,6 +469,7 @@ def main():
     ap.add_argument("--no-scan", action="store_true", help="skip the larger-walker-count sweep scan")
     ap.add_argument("--serving-rates", default="50,500,2000", help="comma-separated Poisson rates (req/s)")
     ap.add_argument("--serving-requests", type=int, default=300)
+    ap.add_argument("--serving-lookahead", type=int, default=1, help="ticks the server loop may run ahead")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -653,7 +654,7 @@ def main():
         res["batched_uneven"] = batched_uneven(precision=args.precision)
     if not args.no_serving:
         rows = serving([float(r) for r in args.serving_rates.split(",")], args.serving_requests,
-                       precision=args.precision)
+                       precision=args.precision, lookahead=args.serving_lookahead)
         res["serving"] = rows
         head = next((r for r in rows if r["rate_req_per_s"] == 500.0), rows[0])
         res["p50_ms"] = head["batched"]["p50_ms"]

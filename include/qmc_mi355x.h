@@ -37,8 +37,10 @@ enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 };
 #define QMC_MAXCONF 8
 /* Plans with more unique configurations per spin (up to QMC_MAXEXC) must be
  * single and double excitations of the ground state (columns 0..n-1); psi, pdf,
- * the local energy and the Metropolis sweep then use the excitation
- * (trace-trick) kernels, the other entry points return QMC_ERR_UNSUPPORTED. */
+ * the local energy, the Metropolis sweep, the drift, HMC and the log|psi|
+ * adjoint (qmc_logpsi_grad, qmc_logpsi_grad_noalloc) then use the excitation
+ * (trace-trick) kernels; the local-energy adjoint, the row kernels and the
+ * batched launches return QMC_ERR_UNSUPPORTED. */
 #define QMC_MAXEXC 1024
 
 /*
@@ -81,7 +83,7 @@ typedef struct QmcSystemDesc {
     const double* jastrow_atoms; /* [natom*3] e-n Jastrow centres (mol.atom_coords) */
     int32_t nmo;                 /* MO columns used (highest occupied index + 1) */
     int32_t nci;
-    int32_t nconf_up, nconf_dn;  /* unique configurations per spin (<= QMC_MAXCONF) */
+    int32_t nconf_up, nconf_dn;  /* unique configurations per spin (<= QMC_MAXEXC) */
     const int32_t* conf_up;      /* [nconf_up * nup] MO columns */
     const int32_t* conf_dn;      /* [nconf_dn * ndown] */
     const int32_t* ci_up;        /* [nci] unique up configuration of CI term n */

@@ -583,10 +583,10 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_reduce_kernel(ERedArgs a) {
 // owns columns [64 x, 64 x + 64) of exactly those slots, so its in-place
 // write races with no other block; its four wavefronts take every fourth slot
 // of the partition and their sums are combined in a fixed order.
-// slot_reduce_kernel, one thread per output: dW[mu][m] over the AO entries of
-// mu, the other outputs directly, exponent terms (entry i, primitive q) to
-// bas_exp index gao_src[i][q] (the plan's inverse maps list both), each
-// summed over the nfold folded slots.
+// slot_reduce_kernel, one wavefront per output, lane s reading folded slot s:
+// dW[mu][m] over the AO entries of mu, the other outputs directly, exponent
+// terms (entry i, primitive q) to bas_exp index gao_src[i][q] (the plan's
+// inverse maps list both), the 64 lanes summed by a fixed xor butterfly.
 struct RedOut {
     double* grad_w;    // [nao][nmo]
     double* grad_ci;   // [nci]
@@ -595,7 +595,10 @@ struct RedOut {
 };
 
 constexpr int kFoldBlock = 256;  // four wavefronts per block of slot_fold_kernel
-constexpr int kMaxFold = 64;     // folded slots left for slot_reduce_kernel
+
+inline dim3 grid_for_waves_of(int nwaves) { return dim3((unsigned)((nwaves + kBlockWaves - 1) / kBlockWaves)); }
+constexpr int kMaxFold = 64;     // folded slots left for slot_reduce_kernel (one per lane)
+static_assert(kMaxFold <= 64, "slot_reduce_kernel reads one folded slot per lane");
 
 inline int red_nfold(const RedSlots& r) { return r.nslot < kMaxFold ? r.nslot : kMaxFold; }
 
@@ -623,33 +626,33 @@ static __global__ __launch_bounds__(kFoldBlock) void slot_fold_kernel(RedSlots r
 static __global__ __launch_bounds__(kBlock) void slot_reduce_kernel(Tables t, RedSlots r, RedOut o, int nfold) {
     const int n_w = t.nao * t.nmo;
     const int n_all = n_w + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
-    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n_all; idx += gridDim.x * blockDim.x) {
-        double acc = 0.0;
-        if (idx < n_w) {
-            const int mu = idx / t.nmo, m = idx - mu * t.nmo;
-            const int e0 = t.ao_ptr[mu], e1 = t.ao_ptr[mu + 1];
-            for (int e = e0; e < e1; ++e) {
-                const int col = t.ao_ent[e] * t.nmo + m;
-                for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + col];
-            }
+    const int idx = global_wave();  // one wavefront per output, lane s reads folded slot s
+    if (idx >= n_all) return;
+    const int lane = threadIdx.x & 63;
+    const bool mine = lane < nfold;
+    const double* ws = r.ws + (size_t)(mine ? lane : 0) * r.stride;
+    double acc = 0.0;
+    if (idx < n_w) {
+        const int mu = idx / t.nmo, m = idx - mu * t.nmo;
+        for (int e = t.ao_ptr[mu]; e < t.ao_ptr[mu + 1]; ++e) acc += mine ? ws[t.ao_ent[e] * t.nmo + m] : 0.0;
+    } else if (idx < n_w + t.nci) {
+        acc = mine ? ws[r.off_ci + idx - n_w] : 0.0;
+    } else if (idx < n_w + t.nci + 2) {
+        acc = mine ? ws[r.off_jas + idx - n_w - t.nci] : 0.0;
+    } else {
+        const int src = idx - n_w - t.nci - 2;  // bas_exp index
+        for (int c = t.exp_ptr[src]; c < t.exp_ptr[src + 1]; ++c) acc += mine ? ws[r.off_exp + t.exp_col[c]] : 0.0;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        if (idx < n_w)
             o.grad_w[idx] = acc;
-        } else if (idx < n_w + t.nci) {
-            const int n = idx - n_w;
-            for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + r.off_ci + n];
-            o.grad_ci[n] = acc;
-        } else if (idx < n_w + t.nci + 2) {
-            const int k = idx - n_w - t.nci;
-            for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + r.off_jas + k];
-            o.grad_jas[k] = acc;
-        } else {
-            const int src = idx - n_w - t.nci - 2;  // bas_exp index
-            const int c0 = t.exp_ptr[src], c1 = t.exp_ptr[src + 1];
-            for (int c = c0; c < c1; ++c) {
-                const int col = r.off_exp + t.exp_col[c];
-                for (int s = 0; s < nfold; ++s) acc += r.ws[(size_t)s * r.stride + col];
-            }
-            o.grad_exp[src] = acc;
-        }
+        else if (idx < n_w + t.nci)
+            o.grad_ci[idx - n_w] = acc;
+        else if (idx < n_w + t.nci + 2)
+            o.grad_jas[idx - n_w - t.nci] = acc;
+        else
+            o.grad_exp[idx - n_w - t.nci - 2] = acc;
     }
 }
 
@@ -661,8 +664,7 @@ inline void red_slot_sum(const Tables& t, const RedSlots& r, const RedOut& o, hi
         hipLaunchKernelGGL(slot_fold_kernel, dim3((unsigned)((r.stride + 63) / 64), (unsigned)nfold), dim3(kFoldBlock),
                            0, s, r, nfold);
     const int n = t.nao * t.nmo + t.nci + 2 + (o.grad_exp ? t.nprim_ref : 0);
-    hipLaunchKernelGGL(slot_reduce_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, t, r, o,
-                       nfold);
+    hipLaunchKernelGGL(slot_reduce_kernel, grid_for_waves_of(n), dim3(kBlock), 0, s, t, r, o, nfold);
 }
 
 // Layout of a slot and the slot count for nw walkers within ws_bytes.
