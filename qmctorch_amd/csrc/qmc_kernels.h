@@ -1034,8 +1034,12 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     }
 }
 
+// The 16- and 24-orbital buckets (one or two walkers per wavefront) are held to
+// two waves per SIMD, so a second wavefront covers the latency of the FP64
+// elimination; left alone the compiler keeps VGPRs and AGPRs for one.
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__global__ __launch_bounds__(kBlock) void sweep_kernel(SArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM >= 16 ? 2 : 1))) void sweep_kernel(
+    SArgs a) {
     const WalkerLane L = lane_layout(a.t, a.nw);
     if ((int64_t)L.wave * L.G >= a.nw) return;
     sweep_walkers<T, NM, RAD, AO>(a, L);
@@ -1078,7 +1082,8 @@ struct SJob {
 };
 
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
-__global__ __launch_bounds__(kBlock) void sweep_batch_kernel(const SJob* __restrict__ jobs, int njobs, int nwaves) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM >= 16 ? 2 : 1))) void sweep_batch_kernel(
+    const SJob* __restrict__ jobs, int njobs, int nwaves) {
     const int wave = global_wave();
     if (wave >= nwaves) return;
     const int j = find_job(jobs, njobs, wave);
