@@ -66,12 +66,25 @@ def uneven_batch(nsys: int = 64, seed: int = 0, wmin: int = 256, wmax: int = 819
     return [(names[i], int(n)) for i, n in zip(idx, w)]
 
 
+def _electron_centers(mol, device) -> torch.Tensor:
+    """Nucleus (i mod Nat) of every electron i, kept on the device with the
+    molecule: a host-to-device copy per request would wait for the stream and
+    stall the serving loop behind the running tick."""
+    key = str(torch.device(device))
+    cache = mol.__dict__.setdefault("_electron_centers", {})
+    c = cache.get(key)
+    if c is None:
+        at = np.asarray(mol.atom_coords, dtype=np.float64).reshape(-1, 3)
+        c = torch.as_tensor(at[np.arange(mol.nelec) % len(at)], device=device)
+        cache[key] = c
+    return c
+
+
 def initial_positions(mol, nwalkers: int, seed: int, sigma: float = 1.0, device="cuda") -> torch.Tensor:
     """Electron i ~ N(nucleus (i mod Nat), sigma^2 I), drawn on the device from
     a generator seeded with ``seed`` (independent of any other draw)."""
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
-    at = torch.as_tensor(np.asarray(mol.atom_coords, dtype=np.float64), device=device)
-    centers = at[torch.arange(mol.nelec, device=device) % at.shape[0]]
+    centers = _electron_centers(mol, device)
     z = torch.randn(nwalkers, mol.nelec, 3, generator=g, dtype=torch.float64, device=device)
     return (centers[None] + sigma * z).reshape(nwalkers, 3 * mol.nelec).contiguous()
