@@ -432,22 +432,6 @@ __device__ __forceinline__ void lane_gauss_jordan(double (&a)[NM], double (&b)[N
     }
 }
 
-// Sum of v over the lanes [base, base+cnt) (result in every lane; cnt uniform bound).
-// Terms are fetched eight at a time (independent shuffles) and added in lane order.
-template <typename V>
-__device__ __forceinline__ V seg_sum(V v, int base, int cnt, int cnt_max) {
-    V s = V(0);
-    for (int t0 = 0; t0 < cnt_max; t0 += 8) {
-        V x[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) x[u] = __shfl(v, base + (t0 + u < cnt ? t0 + u : 0));
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (t0 + u < cnt) s += x[u];
-    }
-    return s;
-}
-
 // ------------------------------------------------------------ Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
 // SC'11.  Round constants and Weyl key increments from that paper.
@@ -460,8 +444,11 @@ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32) instead
+        // of separate high and low multiplies
+        const uint64_t p0 = (uint64_t)M0 * (uint64_t)c.x, p1 = (uint64_t)M1 * (uint64_t)c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += W0;
         k1 += W1;

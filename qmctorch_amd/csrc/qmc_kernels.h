@@ -19,6 +19,28 @@ namespace qmc {
 constexpr int kBlock = 256;
 constexpr int kBlockWaves = kBlock / 64;
 
+// Sum of v over the lanes [base, base+cnt) of this wavefront, in every lane
+// (cnt may differ between groups, cnt_max is its wave-uniform bound): each
+// lane stores v in the wavefront's LDS slot block and adds its group's slots
+// in lane order (LDS loads instead of a shuffle per term).
+template <typename V>
+__device__ __forceinline__ V* wave_sumbuf() {
+    __shared__ V buf[kBlockWaves * 64];
+    return buf + (threadIdx.x >> 6) * 64;
+}
+
+template <typename V>
+__device__ __forceinline__ V seg_sum(V v, int base, int cnt, int cnt_max) {
+    V* B = wave_sumbuf<V>();
+    wave_lds_sync();  // the previous sum's loads are done
+    B[threadIdx.x & 63] = v;
+    wave_lds_sync();
+    V s = V(0);
+    for (int u = 0; u < cnt_max; ++u)
+        if (u < cnt) s += B[base + u];
+    return s;
+}
+
 // Dynamic LDS of the kernels that stage rows per block (W rows of generated
 // sweeps, the excitation kernels' M / X / Mk rows).
 extern __shared__ __attribute__((aligned(16))) unsigned char qmc_dyn_lds[];
@@ -223,6 +245,14 @@ struct MfmaSink {
     }
 };
 
+// Electron positions of this wavefront for the Jastrow pair loop: lane l's
+// (x, y, z) at [l][0..2] in the bulk precision.
+template <typename T>
+__device__ __forceinline__ T* wave_posbuf() {
+    __shared__ __attribute__((aligned(16))) T pos[kBlockWaves * 64 * 4];
+    return pos + (threadIdx.x >> 6) * 64 * 4;
+}
+
 // Pade Jastrow terms of one electron (jastrow_factor_electron_electron.py:124-260,
 // jastrow_factor_electron_nuclei.py:56-161, pade kernels): value contributions,
 // gradient of ln J with respect to this electron and its Laplacian of ln J;
@@ -237,21 +267,20 @@ __device__ __forceinline__ void jastrow_and_potential(const Tables& t, const Qmc
     const int Ne = t.nelec;
     const double wee = (t.use_ee && prm.w_ee) ? ldc(prm.w_ee, 0) : 0.0;
     const double wen = (t.use_en && prm.w_en) ? ldc(prm.w_en, 0) : 0.0;
-    // partner coordinates fetched four electrons at a time (independent shuffles)
-    for (int j0 = 0; j0 < Ne; j0 += 4) {
-        double px[4], py[4], pz[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int jl = L.wbase + (j0 + u < Ne ? j0 + u : 0);
-            px[u] = __shfl(x, jl);
-            py[u] = __shfl(y, jl);
-            pz[u] = __shfl(z, jl);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = j0 + u;
-            const double xj = px[u], yj = py[u], zj = pz[u];
-            if (j < Ne && j != L.e) {
+    // partner coordinates through the wavefront's LDS block: one store of this
+    // lane's position, one vector load per partner
+    double* P = wave_posbuf<double>();
+    wave_lds_sync();  // the previous evaluation's loads are done
+    P[L.lane * 4 + 0] = x;
+    P[L.lane * 4 + 1] = y;
+    P[L.lane * 4 + 2] = z;
+    wave_lds_sync();
+    {
+#pragma unroll 4
+        for (int j = 0; j < Ne; ++j) {
+            const double* q = P + (L.wbase + j) * 4;
+            const double xj = q[0], yj = q[1], zj = q[2];
+            if (j != L.e) {
                 const double dx = x - xj, dy = y - yj, dz = z - zj;
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 const double ir = 1.0 / r;
@@ -313,30 +342,24 @@ __device__ __forceinline__ T jastrow_value(const Tables& t, const QmcParams& prm
         const T xs = (T)x, ys = (T)y, zs = (T)z;
         const int Ne = t.nelec, H = t.nelec / 2;
         const bool even = (Ne & 1) == 0;
-        // partner coordinates fetched four at a time (independent shuffles)
-        for (int d0 = 1; d0 <= H; d0 += 4) {
-            T xj[4], yj[4], zj[4];
-            int jj[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int d = d0 + u <= H ? d0 + u : 1;
-                const int j = L.e + d < Ne ? L.e + d : L.e + d - Ne;
-                jj[u] = j;
-                xj[u] = __shfl(xs, L.wbase + j);
-                yj[u] = __shfl(ys, L.wbase + j);
-                zj[u] = __shfl(zs, L.wbase + j);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int d = d0 + u;
-                if (d <= H) {
-                    const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
-                    const T r = fsqrt(dx * dx + dy * dy + dz * dz);
-                    const T w0 = ((jj[u] < t.nup) == (L.e < t.nup)) ? T(0.25) : T(0.5);
-                    const T wt = (even && d == H) ? T(0.5) * w0 : w0;
-                    k += wt * r * frcp(T(1) + wee * r);
-                }
-            }
+        // partner positions through the wavefront's LDS block: one store of
+        // this lane's position, one vector load per partner (independent loads
+        // instead of three shuffles each)
+        T* P = wave_posbuf<T>();
+        wave_lds_sync();  // the previous evaluation's loads are done
+        P[L.lane * 4 + 0] = xs;
+        P[L.lane * 4 + 1] = ys;
+        P[L.lane * 4 + 2] = zs;
+        wave_lds_sync();
+#pragma unroll 5
+        for (int d = 1; d <= H; ++d) {
+            const int j = L.e + d < Ne ? L.e + d : L.e + d - Ne;
+            const T* q = P + (L.wbase + j) * 4;
+            const T dx = xs - q[0], dy = ys - q[1], dz = zs - q[2];
+            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+            const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.25) : T(0.5);
+            const T wt = (even && d == H) ? T(0.5) * w0 : w0;
+            k += wt * r * frcp(T(1) + wee * r);
         }
     }
     if (t.use_en) {
@@ -1127,31 +1150,26 @@ __device__ __forceinline__ T jastrow_value_grad(const Tables& t, const QmcParams
     if (t.use_ee) {
         const T wee = (T)ldc(prm.w_ee, 0);
         const T xs = (T)x, ys = (T)y, zs = (T)z;
-        for (int j0 = 0; j0 < t.nelec; j0 += 4) {
-            T xj[4], yj[4], zj[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int jl = L.wbase + (j0 + u < t.nelec ? j0 + u : 0);
-                xj[u] = __shfl(xs, jl);
-                yj[u] = __shfl(ys, jl);
-                zj[u] = __shfl(zs, jl);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int j = j0 + u;
-                if (j < t.nelec) {
-                    const T dx = xs - xj[u], dy = ys - yj[u], dz = zs - zj[u];
-                    const T r = fsqrt(dx * dx + dy * dy + dz * dz);
-                    const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
-                    const T den = frcp(T(1) + wee * r);
-                    const bool other = j != L.e;
-                    k += other ? w0 * r * den : T(0);
-                    const T c = other ? T(2) * w0 * den * den * frcp(r) : T(0);
-                    g[0] += c * dx;
-                    g[1] += c * dy;
-                    g[2] += c * dz;
-                }
-            }
+        // partner positions through the wavefront's LDS block (as jastrow_value)
+        T* P = wave_posbuf<T>();
+        wave_lds_sync();
+        P[L.lane * 4 + 0] = xs;
+        P[L.lane * 4 + 1] = ys;
+        P[L.lane * 4 + 2] = zs;
+        wave_lds_sync();
+#pragma unroll 4
+        for (int j = 0; j < t.nelec; ++j) {
+            const T* q = P + (L.wbase + j) * 4;
+            const T dx = xs - q[0], dy = ys - q[1], dz = zs - q[2];
+            const T r = fsqrt(dx * dx + dy * dy + dz * dz);
+            const T w0 = ((j < t.nup) == (L.e < t.nup)) ? T(0.125) : T(0.25);  // half of 0.25 / 0.5
+            const T den = frcp(T(1) + wee * r);
+            const bool other = j != L.e;
+            k += other ? w0 * r * den : T(0);
+            const T c = other ? T(2) * w0 * den * den * frcp(r) : T(0);
+            g[0] += c * dx;
+            g[1] += c * dy;
+            g[2] += c * dz;
         }
     }
     if (t.use_en) {
