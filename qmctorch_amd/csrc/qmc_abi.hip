@@ -212,7 +212,7 @@ int refuse_excitations(const QmcPlan* plan) {
     if (plan && plan->t.exc_mode)
         return fail(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin: this entry point has no "
                                          "excitation (trace-trick) path (psi, pdf, local energy, the Metropolis "
-                                         "sweep, the drift, HMC and the log|psi| adjoint have)");
+                                         "sweep, the drift, HMC and the log|psi| and local-energy adjoints have)");
     return QMC_OK;
 }
 
@@ -614,7 +614,6 @@ int qmc_eloc_grad(const QmcPlan* plan, const QmcParams* params, const double* po
                   qmc_stream_t stream) {
     int rc = check_common(plan, params, nw);
     if (rc) return rc;
-    if ((rc = refuse_excitations(plan))) return rc;
     if (!out || !out->grad_w || !out->grad_ci || !out->grad_jas) return fail(QMC_ERR_ARG, "null gradient output");
     if (out->grad_exp) return fail(QMC_ERR_UNSUPPORTED, "d E_L / d bas_exp is not provided");
     const RedSlots r = red_layout(plan->t, nw, false, (double*)workspace, workspace_bytes, weights);
@@ -637,7 +636,15 @@ int qmc_eloc_grad(const QmcPlan* plan, const QmcParams* params, const double* po
     a.nw = nw;
     a.eloc = eloc;
     a.r = r;
-    if (QMC_BY_RAD(launch_eloc_reduce, a.t.radial, a, o, s)) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    if (a.t.exc_mode) {
+        const int miss = QMC_BY_RAD(launch_eloc_reduce_exc, a.t.radial, a, o, s);
+        if (miss == -2)
+            return fail(QMC_ERR_UNSUPPORTED, "the local-energy adjoint of this excitation plan needs more LDS per "
+                                             "wavefront than a block can hold (64 KiB of row blocks at most)");
+        if (miss) return fail(QMC_ERR_UNSUPPORTED, kNoExcBucket);
+    } else if (QMC_BY_RAD(launch_eloc_reduce, a.t.radial, a, o, s)) {
+        return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
+    }
     QMC_HIP(hipGetLastError());
     return QMC_OK;
 }

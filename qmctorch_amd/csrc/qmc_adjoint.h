@@ -385,9 +385,10 @@ struct ERedArgs {
 
 // Adds g * d E_L / d theta of the wavefront's walkers to its slot
 // (eloc_grad_kernel's quantities, contracted instead of written).
-template <int NM, int RAD>
+// EXC: plans with exc_mode (exc_eloc_gamma's rows in xlds).
+template <int NM, int RAD, bool EXC = false>
 __device__ __forceinline__ void eloc_adjoint_reduce(const ERedArgs& a, const WalkerLane& L, double* slot,
-                                                    double* stage) {
+                                                    double* stage, double* xlds = nullptr) {
     const Tables& t = a.t;
     const double* p = a.pos + (size_t)L.wclamp * 3 * t.nelec + 3 * L.e;
     const double x = p[0], y = p[1], z = p[2];
@@ -424,98 +425,115 @@ __device__ __forceinline__ void eloc_adjoint_reduce(const ERedArgs& a, const Wal
     for (int m = 0; m < NM; ++m)
         bk[m] = -0.5 * (lp[m] + 2.0 * (g[0] * gx[m] + g[1] * gy[m] + g[2] * gz[m]) + l2 * mo[m]);
 
-    double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF];
-#pragma unroll
-    for (int c = 0; c < QMC_MAXCONF; ++c) {
-        ld[c] = 0.0;
-        sd[c] = 1.0;
-        td[c] = 0.0;
-    }
-#pragma unroll 1
-    for (int c = 0; c < t.nconf_max; ++c) {
-        double aa[NM], bb[NM];
-        gather_conf<NM>(t, L, c, mo, aa);
-        gather_conf<NM>(t, L, c, bk, bb);
-        double lg, sg, tp;
-        int col;
-        lane_gj<NM>(L, t.nsig_max, aa, bb, lg, sg, tp, col);
-        const double tr = seg_sum(tp, L.gbase, L.n_sig, t.nsig_max);
-#pragma unroll
-        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
-            if (c2 == c) {
-                ld[c2] = lg;
-                sd[c2] = sg;
-                td[c2] = tr;
-            }
-    }
-    CiTerms ct;
-    ci_terms(t, a.p, L, ld, sd, td, ct);
-    double ST = 0.0;
-    for (int n = 0; n < t.nci; ++n) {
-        double lg, sg, tr;
-        ct.term(t, n, lg, sg, tr);
-        ST += ldc(a.p.ci, n) * sg * exp(lg - ct.lmax) * tr;
-    }
-    const double T = ST / ct.S;
-    double rho[QMC_MAXCONF], om[QMC_MAXCONF];
-#pragma unroll
-    for (int c = 0; c < QMC_MAXCONF; ++c) rho[c] = om[c] = 0.0;
-    for (int n = 0; n < t.nci; ++n) {
-        double lg, sg, tr;
-        ct.term(t, n, lg, sg, tr);
-        const double dns = sg * exp(lg - ct.lmax) / ct.S;
-        const double wn = ldc(a.p.ci, n) * dns;
-        const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
-#pragma unroll
-        for (int c = 0; c < QMC_MAXCONF; ++c)
-            if (c == mine) {
-                rho[c] += wn * (tr - T);
-                om[c] += wn;
-            }
-        const double sum = wave_sum((v && L.e == 0) ? gw * dns * (tr - T) : 0.0);
-        const int o = a.r.off_ci + n;
-        if (red_owner(o) == L.lane) slot[o] += sum;
-    }
     double GA[NM], GB[NM];
-#pragma unroll
-    for (int m = 0; m < NM; ++m) GA[m] = GB[m] = 0.0;
     double s[3] = {0.0, 0.0, 0.0};
-#pragma unroll 1
-    for (int c = 0; c < t.nconf_max; ++c) {
-        double aa[NM], xb[NM], ib[NM];
-        gather_conf<NM>(t, L, c, mo, aa);
-        gather_conf<NM>(t, L, c, bk, xb);
-        double lg, sg, tp;
-        int col, col2;
-        lane_gj<NM>(L, t.nsig_max, aa, xb, lg, sg, tp, col);
-        gather_conf<NM>(t, L, c, mo, aa);
+    double T;
+    if constexpr (EXC) {
+        T = exc_eloc_gamma<NM>(t, a.p, L, mo, bk, xlds, GA, GB, [&](int nn, double val) {
+            const double sum = wave_sum((v && L.e == 0) ? gw * val : 0.0);
+            const int o = a.r.off_ci + nn;
+            if (red_owner(o) == L.lane) slot[o] += sum;
+        });
 #pragma unroll
-        for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
-        lane_gj<NM>(L, t.nsig_max, aa, ib, lg, sg, tp, col2);
-        double inv[NM], zz[NM];
-        inverse_columns<NM>(L, t.nsig_max, col, xb, ib, inv, zz);
-        double rc = 0.0, oc = 0.0;
-#pragma unroll
-        for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
-            if (c2 == c) {
-                rc = rho[c2];
-                oc = om[c2];
+        for (int m = 0; m < NM; ++m) {
+            if (m < t.nmo) {
+                s[0] += GB[m] * gx[m];
+                s[1] += GB[m] * gy[m];
+                s[2] += GB[m] * gz[m];
             }
-        const QMC_CONST int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
-        const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
-        if (c < nconf) {
+        }
+    } else {
+        double ld[QMC_MAXCONF], sd[QMC_MAXCONF], td[QMC_MAXCONF];
 #pragma unroll
-            for (int k = 0; k < NM; ++k) {
-                if (k < L.n_sig) {
-                    const int mcol = t.identity_conf ? k : conf[k];
+        for (int c = 0; c < QMC_MAXCONF; ++c) {
+            ld[c] = 0.0;
+            sd[c] = 1.0;
+            td[c] = 0.0;
+        }
+#pragma unroll 1
+        for (int c = 0; c < t.nconf_max; ++c) {
+            double aa[NM], bb[NM];
+            gather_conf<NM>(t, L, c, mo, aa);
+            gather_conf<NM>(t, L, c, bk, bb);
+            double lg, sg, tp;
+            int col;
+            lane_gj<NM>(L, t.nsig_max, aa, bb, lg, sg, tp, col);
+            const double tr = seg_sum(tp, L.gbase, L.n_sig, t.nsig_max);
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) {
-                        if (m == mcol) {
-                            GA[m] += rc * inv[k] - oc * zz[k];
-                            GB[m] += oc * inv[k];
-                            s[0] += oc * inv[k] * gx[m];
-                            s[1] += oc * inv[k] * gy[m];
-                            s[2] += oc * inv[k] * gz[m];
+            for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+                if (c2 == c) {
+                    ld[c2] = lg;
+                    sd[c2] = sg;
+                    td[c2] = tr;
+                }
+        }
+        CiTerms ct;
+        ci_terms(t, a.p, L, ld, sd, td, ct);
+        double ST = 0.0;
+        for (int n = 0; n < t.nci; ++n) {
+            double lg, sg, tr;
+            ct.term(t, n, lg, sg, tr);
+            ST += ldc(a.p.ci, n) * sg * exp(lg - ct.lmax) * tr;
+        }
+        T = ST / ct.S;
+        double rho[QMC_MAXCONF], om[QMC_MAXCONF];
+#pragma unroll
+        for (int c = 0; c < QMC_MAXCONF; ++c) rho[c] = om[c] = 0.0;
+        for (int n = 0; n < t.nci; ++n) {
+            double lg, sg, tr;
+            ct.term(t, n, lg, sg, tr);
+            const double dns = sg * exp(lg - ct.lmax) / ct.S;
+            const double wn = ldc(a.p.ci, n) * dns;
+            const int mine = L.spin == 0 ? t.ci_up[n] : t.ci_dn[n];
+#pragma unroll
+            for (int c = 0; c < QMC_MAXCONF; ++c)
+                if (c == mine) {
+                    rho[c] += wn * (tr - T);
+                    om[c] += wn;
+                }
+            const double sum = wave_sum((v && L.e == 0) ? gw * dns * (tr - T) : 0.0);
+            const int o = a.r.off_ci + n;
+            if (red_owner(o) == L.lane) slot[o] += sum;
+        }
+#pragma unroll
+        for (int m = 0; m < NM; ++m) GA[m] = GB[m] = 0.0;
+#pragma unroll 1
+        for (int c = 0; c < t.nconf_max; ++c) {
+            double aa[NM], xb[NM], ib[NM];
+            gather_conf<NM>(t, L, c, mo, aa);
+            gather_conf<NM>(t, L, c, bk, xb);
+            double lg, sg, tp;
+            int col, col2;
+            lane_gj<NM>(L, t.nsig_max, aa, xb, lg, sg, tp, col);
+            gather_conf<NM>(t, L, c, mo, aa);
+#pragma unroll
+            for (int j = 0; j < NM; ++j) ib[j] = (j == L.myrow) ? 1.0 : 0.0;
+            lane_gj<NM>(L, t.nsig_max, aa, ib, lg, sg, tp, col2);
+            double inv[NM], zz[NM];
+            inverse_columns<NM>(L, t.nsig_max, col, xb, ib, inv, zz);
+            double rc = 0.0, oc = 0.0;
+#pragma unroll
+            for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
+                if (c2 == c) {
+                    rc = rho[c2];
+                    oc = om[c2];
+                }
+            const QMC_CONST int* conf = L.spin == 0 ? t.conf_up + c * t.nup : t.conf_dn + c * t.ndown;
+            const int nconf = L.spin == 0 ? t.nconf_up : t.nconf_dn;
+            if (c < nconf) {
+#pragma unroll
+                for (int k = 0; k < NM; ++k) {
+                    if (k < L.n_sig) {
+                        const int mcol = t.identity_conf ? k : conf[k];
+#pragma unroll
+                        for (int m = 0; m < NM; ++m) {
+                            if (m == mcol) {
+                                GA[m] += rc * inv[k] - oc * zz[k];
+                                GB[m] += oc * inv[k];
+                                s[0] += oc * inv[k] * gx[m];
+                                s[1] += oc * inv[k] * gy[m];
+                                s[2] += oc * inv[k] * gz[m];
+                            }
                         }
                     }
                 }
@@ -574,6 +592,22 @@ __global__ __launch_bounds__(kBlock) void eloc_grad_reduce_kernel(ERedArgs a) {
     for (int64_t vw = slot_id; vw < a.r.nwaves; vw += a.r.nslot) {
         const WalkerLane L = lane_layout_w(a.t, a.nw, (int)vw);
         eloc_adjoint_reduce<NM, RAD>(a, L, slot, stage);
+    }
+}
+
+// Plans with exc_mode: one wavefront per block, exc_eloc_gamma's rows in the
+// dynamic LDS (2 (G + 1) groups of 6 x nsig_max x NM doubles), one slot each.
+template <int NM, int RAD>
+__global__ __launch_bounds__(64) void eloc_grad_reduce_kernel_exc(ERedArgs a) {
+    const int slot_id = blockIdx.x;
+    if (slot_id >= a.r.nslot) return;
+    const int lane = threadIdx.x & 63;
+    double* slot = a.r.ws + (size_t)slot_id * a.r.stride;
+    red_zero_slot(a.r, a.t.nentry, a.t.nmo, slot_id, lane);
+    __shared__ double stage[2 * 64 * (kRedCols + 1)];
+    for (int64_t vw = slot_id; vw < a.r.nwaves; vw += a.r.nslot) {
+        const WalkerLane L = lane_layout_w(a.t, a.nw, (int)vw);
+        eloc_adjoint_reduce<NM, RAD, true>(a, L, slot, stage, (double*)qmc_dyn_lds);
     }
 }
 

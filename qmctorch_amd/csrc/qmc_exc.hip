@@ -84,4 +84,40 @@ int launch_grad_reduce_exc<QMC_RAD>(const GRedArgs& a, const RedOut& o, hipStrea
     return 0;
 }
 
+// static LDS of a kernel plus dynamic bytes within the device's per-block limit
+template <typename K>
+static bool lds_fits(K kernel, size_t dynamic) {
+    hipFuncAttributes fa;
+    int dev = 0, limit = 0;
+    if (hipFuncGetAttributes(&fa, (const void*)kernel) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return dynamic <= kExcEnergyAdjointLds;
+    return fa.sharedSizeBytes + dynamic <= (size_t)limit;
+}
+
+template <int NM, int RAD>
+static int go_eloc_reduce(const ERedArgs& a, hipStream_t s) {
+    const size_t lds = 2 * exc_lds_bytes(a.t, NM);  // six row blocks per group
+    if (lds > kExcEnergyAdjointLds || !lds_fits(eloc_grad_reduce_kernel_exc<NM, RAD>, lds)) return -2;
+    hipLaunchKernelGGL((eloc_grad_reduce_kernel_exc<NM, RAD>), dim3((unsigned)a.r.nslot), dim3(64), lds, s, a);
+    return 0;
+}
+
+template <>
+int launch_eloc_reduce_exc<QMC_RAD>(const ERedArgs& a, const RedOut& o, hipStream_t s) {
+    int rc = -1;
+    switch (nm_bucket_fine(a.t)) {
+#define QMC_CASE(K)                             \
+    case K:                                     \
+        rc = go_eloc_reduce<K, QMC_RAD>(a, s);  \
+        break;
+        QMC_EXC_BUCKETS(QMC_CASE)
+#undef QMC_CASE
+        default: return -1;
+    }
+    if (rc) return rc;
+    red_slot_sum(a.t, a.r, o, s);
+    return 0;
+}
+
 }  // namespace qmc

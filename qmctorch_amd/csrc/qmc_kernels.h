@@ -804,6 +804,237 @@ __device__ __forceinline__ void exc_gamma(const Tables& t, const QmcParams& prm,
     }
 }
 
+// Local-energy adjoint rows for plans with exc_mode (the adjoint of
+// slater_exc's kinetic ratio): with S = sum_n c_n R_n^up R_n^dn and
+// T = (1/S) sum_n c_n (Q_n^up R_n^dn + R_n^up Q_n^dn) (the kinetic term; B rows
+// carry the -1/2 and the Jastrow terms),
+//   dT = (1/S) sum_n c_n [R_n^s' dQ_n^s + (Q_n^s' - T R_n^s') dR_n^s]   (s' the other spin),
+// and R, Q depend on M = A^-1 V, Mk = X_virt - X_occ M and t0 = tr X_occ
+// (X = A^-1 B).  With G_M, G_K, g_t0 the adjoints of M, Mk, t0 (from dR/dM,
+// dQ/dM, dQ/dMk, dQ/dt0 of every term that touches the hole),
+//   Gbar_X[h][n+p] = G_K[h][p],  Gbar_X[h][j] = -sum_p G_K[h][p] M[j][p] + [h = j] g_t0,
+//   Gbar_M[j][p]   = G_M[j][p] - sum_h X[h][j] G_K[h][p],
+// and through X = A^-1 B, M = A^-1 V:
+//   GB[i][:] = sum_h (A^-1)[h][i] Gbar_X[h][:],
+//   GA[i][:] = sum_h (A^-1)[h][i] ZA[h][:],  ZA[h][k] = -(Gbar_X X^T + Gbar_M M^T)[h][k],
+//                                            ZA[h][n+p] = Gbar_M[h][p],
+// GA and GB the coefficients of this electron's MO row and kinetic row.
+// lds: 2 (G + 1) groups of six nsig_max x NM blocks (M, X, Mk then G_K, A^-1,
+// ZA, ZB).  ci_out(n, dT/dc_n) is called by every lane for every CI term.
+// Returns T.
+template <int NM, typename CiOut>
+__device__ __forceinline__ double exc_eloc_gamma(const Tables& t, const QmcParams& prm, const WalkerLane& L,
+                                                 const double (&mo)[NM], const double (&bk)[NM], double* lds,
+                                                 double (&GA)[NM], double (&GB)[NM], CiOut&& ci_out) {
+    const int n = L.n_sig;
+    const int blk = t.nsig_max * NM;
+    double* Mb = lds + (size_t)(L.wl * 2 + L.spin) * 6 * blk;
+    double* Xb = Mb + blk;
+    double* Kb = Xb + blk;
+    double* Ab = Kb + blk;
+    double* ZAb = Ab + blk;
+    double* ZBb = ZAb + blk;
+    double a[NM], b[NM];
+    double lg, sg, tp;
+    int col, col2, col3;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? mo[j] : 0.0;
+        b[j] = mo[j];
+    }
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
+    wave_lds_sync();  // the previous walkers' reads are done
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Mb[col * NM + j] = b[j];  // row col of [I | M]
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? mo[j] : 0.0;
+        b[j] = bk[j];
+    }
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col2);
+    const double t0 = seg_sum(tp, L.gbase, n, t.nsig_max);  // tr(A^-1 B_occ)
+    if (L.lane_real && col2 >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Xb[col2 * NM + j] = b[j];  // row col2 of X
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        a[j] = j < n ? mo[j] : 0.0;
+        b[j] = (j == L.myrow) ? 1.0 : 0.0;
+    }
+    lane_gauss_jordan<NM, true>(a, b, n, t.nsig_max, L.myrow, L.gbase, L.gmask, lg, sg, tp, col3);
+    if (L.lane_real && col3 >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) Ab[col3 * NM + j] = b[j];  // row col3 of A^-1
+    }
+    wave_lds_sync();
+    // row col of Mk: X[col][n + p] - sum_j X[col][j] M[j][n + p]
+    if (L.lane_real && col >= 0) {
+        for (int p = 0; n + p < t.nmo; ++p) {
+            double v = Xb[col * NM + n + p];
+            for (int j = 0; j < n; ++j) v -= Xb[col * NM + j] * Mb[j * NM + n + p];
+            Kb[col * NM + p] = v;
+        }
+    }
+    wave_lds_sync();
+    // walker sums S, ST (lane e takes terms e, e + Ne, ...) and T
+    const int up_lane = L.wbase, dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
+    const double tu = __shfl(t0, up_lane), td = __shfl(t0, dn_lane);
+    const double* Mu = lds + (size_t)(L.wl * 2) * 6 * blk;
+    const double* Md = Mu + 6 * blk;
+    const double* Ku = Mu + 2 * blk;
+    const double* Kd = Md + 2 * blk;
+    auto terms = [&](int m, double& Ru, double& Qu, double& Rd, double& Qd) {
+        exc_ratio<NM, true>(t.exc_up + 6 * t.ci_up[m], Mu, Ku, t.nup, tu, Ru, Qu);
+        Rd = 1.0;
+        Qd = 0.0;
+        if (t.ndown > 0) exc_ratio<NM, true>(t.exc_dn + 6 * t.ci_dn[m], Md, Kd, t.ndown, td, Rd, Qd);
+    };
+    double s = 0.0, st = 0.0;
+    for (int m = L.e; m < t.nci; m += t.nelec) {
+        double Ru, Qu, Rd, Qd;
+        terms(m, Ru, Qu, Rd, Qd);
+        const double c = ldc(prm.ci, m);
+        s += c * Ru * Rd;
+        st += c * (Qu * Rd + Ru * Qd);
+    }
+    const double S = seg_sum(s, L.wbase, t.nelec, t.nelec);
+    const double T = seg_sum(st, L.wbase, t.nelec, t.nelec) / S;
+    for (int m = 0; m < t.nci; ++m) {
+        double Ru, Qu, Rd, Qd;
+        terms(m, Ru, Qu, Rd, Qd);
+        ci_out(m, (Qu * Rd + Ru * Qd - T * Ru * Rd) / S);
+    }
+    // rows h = col of G_M and G_K, and g_t0, for this lane's spin
+    const bool up = L.spin == 0;
+    const QMC_CONST int* ex_me = up ? t.exc_up : t.exc_dn;
+    const QMC_CONST int* ex_ot = up ? t.exc_dn : t.exc_up;
+    const double* Mo = up ? Md : Mu;
+    const double* Ko = up ? Kd : Ku;
+    const double t0o = up ? td : tu;
+    const int n_ot = up ? t.ndown : t.nup;
+    const bool other = up ? t.ndown > 0 : true;
+    double gM[NM], gK[NM], gt0 = 0.0;
+#pragma unroll
+    for (int p = 0; p < NM; ++p) gM[p] = gK[p] = 0.0;
+    auto add = [&](double (&g)[NM], int p, double v) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q)
+            if (q == p) g[q] += v;
+    };
+    for (int m = 0; m < t.nci; ++m) {
+        double Ro = 1.0, Qo = 0.0;
+        if (other) exc_ratio<NM, true>(ex_ot + 6 * (up ? t.ci_dn[m] : t.ci_up[m]), Mo, Ko, n_ot, t0o, Ro, Qo);
+        const double c = ldc(prm.ci, m);
+        const QMC_CONST int* e = ex_me + 6 * (up ? t.ci_up[m] : t.ci_dn[m]);
+        const int lvl = e[0];
+        const double sgn_e = (double)e[5];
+        const double al = c * Ro / S * sgn_e, be = c * (Qo - T * Ro) / S * sgn_e;
+        if (lvl == 0) {
+            gt0 += al;
+        } else if (lvl == 1) {
+            const int h1 = e[1], p1 = e[3];
+            gt0 += al * Mb[h1 * NM + n + p1];
+            if (h1 == col) {
+                add(gM, p1, be + al * t0);
+                add(gK, p1, al);
+            }
+        } else {
+            const int h1 = e[1], h2 = e[2], p1 = e[3], p2 = e[4];
+            const double t11 = Mb[h1 * NM + n + p1], t12 = Mb[h1 * NM + n + p2];
+            const double t21 = Mb[h2 * NM + n + p1], t22 = Mb[h2 * NM + n + p2];
+            gt0 += al * (t11 * t22 - t12 * t21);
+            if (h1 == col || h2 == col) {
+                const double k11 = Kb[h1 * NM + p1], k12 = Kb[h1 * NM + p2];
+                const double k21 = Kb[h2 * NM + p1], k22 = Kb[h2 * NM + p2];
+                if (h1 == col) {
+                    add(gM, p1, be * t22 + al * (t22 * t0 + k22));
+                    add(gM, p2, -be * t21 - al * (t21 * t0 + k21));
+                    add(gK, p1, al * t22);
+                    add(gK, p2, -al * t21);
+                }
+                if (h2 == col) {
+                    add(gM, p2, be * t11 + al * (t11 * t0 + k11));
+                    add(gM, p1, -be * t12 - al * (t12 * t0 + k12));
+                    add(gK, p2, al * t11);
+                    add(gK, p1, -al * t12);
+                }
+            }
+        }
+    }
+    wave_lds_sync();  // every read of the Mk rows is done
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int p = 0; p < NM; ++p) Kb[col * NM + p] = gK[p];  // now the G_K rows
+    }
+    wave_lds_sync();
+    // Gbar_M row col and Gbar_X row col
+    double gMb[NM], gX[NM];
+#pragma unroll
+    for (int p = 0; p < NM; ++p) gMb[p] = gM[p];
+    for (int h = 0; h < n; ++h) {
+        const double xh = Xb[h * NM + (col >= 0 ? col : 0)];
+#pragma unroll
+        for (int p = 0; p < NM; ++p) gMb[p] -= xh * Kb[h * NM + p];
+    }
+#pragma unroll
+    for (int j = 0; j < NM; ++j) gX[j] = (j == col) ? gt0 : 0.0;
+#pragma unroll
+    for (int p = 0; p < NM; ++p) {
+        if (n + p < t.nmo) {
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                if (j < n) gX[j] -= gK[p] * Mb[j * NM + n + p];
+#pragma unroll
+            for (int j = 0; j < NM; ++j)
+                if (j == n + p) gX[j] = gK[p];
+        }
+    }
+    // ZA row col
+    double za[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) za[j] = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            if (j < t.nmo) v -= gX[j] * Xb[k * NM + j];
+#pragma unroll
+        for (int p = 0; p < NM; ++p)
+            if (n + p < t.nmo) v -= gMb[p] * Mb[k * NM + n + p];
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            if (j == k) za[j] = v;
+    }
+#pragma unroll
+    for (int p = 0; p < NM; ++p) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j)
+            if (n + p < t.nmo && j == n + p) za[j] = gMb[p];
+    }
+    if (L.lane_real && col >= 0) {
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            ZAb[col * NM + j] = za[j];
+            ZBb[col * NM + j] = gX[j];
+        }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < NM; ++j) GA[j] = GB[j] = 0.0;
+    for (int h = 0; h < n; ++h) {
+        const double y = Ab[h * NM + L.myrow];  // (A^-1)[h][i]
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+            GA[j] += y * ZAb[h * NM + j];
+            GB[j] += y * ZBb[h * NM + j];
+        }
+    }
+    return T;
+}
+
 struct KArgs {
     Tables t;
     QmcParams p;

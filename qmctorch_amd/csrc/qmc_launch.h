@@ -57,6 +57,10 @@ template <int RAD> int launch_eloc_reduce(const ERedArgs& a, const RedOut& o, hi
 // buckets up to 16)
 template <int RAD> int launch_grad_reduce_exc(const GRedArgs& a, const RedOut& o, hipStream_t s);
 template <int RAD> int launch_hmc_exc(const HArgs& a, int precision, hipStream_t s);
+// the local-energy adjoint of plans with exc_mode; -2 when its LDS rows
+// (2 (G + 1) groups of 6 x nsig_max x bucket doubles) exceed kExcEnergyAdjointLds
+template <int RAD> int launch_eloc_reduce_exc(const ERedArgs& a, const RedOut& o, hipStream_t s);
+constexpr size_t kExcEnergyAdjointLds = 64 * 1024;
 // excitation (trace-trick) kernels, any precision; register buckets up to 16
 template <int RAD> int launch_wf_exc(const KArgs& a, int precision, int mode, hipStream_t s);
 template <int RAD> int launch_sweep_exc(const SArgs& a, int precision, hipStream_t s);

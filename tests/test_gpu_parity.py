@@ -335,9 +335,11 @@ def test_solver_single_point_and_run():
         assert torch.equal(v.cpu(), last[k].cpu()), k
 
 
-@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22"])
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_adf_sto_sd22",
+                                  "lih_631g_sd46_ee_en", "h2o_ccpvdz_sd46_ee"])
 def test_local_energy_adjoint_vs_oracle_autograd(case):
-    """d(sum_w g_w E_L,w)/d theta: adjoint kernel vs autograd through the oracle's E_L."""
+    """d(sum_w g_w E_L,w)/d theta: adjoint kernel vs autograd through the oracle's E_L
+    (sd46: the excitation path's adjoint against every determinant taken explicitly)."""
     _say("case eloc adjoint", case)
     mol, kw = helpers.CASES[case]()
     wf, orc = helpers.make_pair(mol, **kw)
@@ -425,13 +427,18 @@ def test_solver_run_auto_gradient():
 
 def test_excitation_plans_refuse_entry_points_without_that_path():
     """Plans with more than 8 unique configurations per spin run psi, pdf, the
-    local energy, the sweep, the drift / HMC and the log|psi| adjoint through
-    the excitation kernels; the local-energy adjoint (grad="auto") says that it
-    has no such path."""
+    local energy, the sweep, the drift / HMC and both adjoints through the
+    excitation kernels; the per-walker adjoint rows and the batched launches
+    say that they have no such path."""
     _say("case excitation refusals")
+    from qmctorch_amd.serving import BatchRunner, SystemContext, WalkerSet
+    from qmctorch_amd.wavefunction.slater_jastrow import wf_adjoint_rows
+
     mol, kw = helpers.CASES["lih_631g_sd46_ee_en"]()
     wf, _ = helpers.make_pair(mol, **kw)
     pos = helpers.positions(mol, 8, seed=2).to(DEV)
-    wf.ao.bas_exp.requires_grad_(False)
     with pytest.raises(RuntimeError, match="excitation"):
-        wf.local_energy_autodiff(pos).sum().backward()
+        wf_adjoint_rows(wf, pos)
+    ws = WalkerSet(SystemContext(wf, "lih_sd46"), pos.clone(), seed=1)
+    with pytest.raises(RuntimeError, match="excitation"):
+        BatchRunner(max_jobs=4).sweep([(ws, 2)])

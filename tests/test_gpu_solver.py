@@ -130,7 +130,7 @@ def test_solver_manual_gradient_matches_oracle(case):
     assert checked >= 3
 
 
-@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en"])
+@pytest.mark.parametrize("case", ["h2o_ccpvdz_ee_en", "lih_631g_cas22_ee_en", "lih_631g_sd46_ee_en"])
 def test_solver_auto_gradient_matches_oracle(case):
     """evaluate_grad_auto back-propagates mean(E_L) (loss.py, use_weight off):
     the device adjoint equals autograd through the oracle's local energy."""
