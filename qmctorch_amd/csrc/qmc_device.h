@@ -173,14 +173,18 @@ __device__ __forceinline__ const double* group_exponents<double>(const QmcParams
 // exp(-a r^2) (or exp(-a r)) of the NP primitives of a group whose exponents
 // start at p0 of the gathered exponent list
 template <typename T, int RAD, int NP>
-__device__ __forceinline__ void group_prims(const QmcParams& prm, int p0, T r2, T r, T (&E)[NP], T (&A)[NP]) {
-    const T* gexp = group_exponents<T>(prm);
+__device__ __forceinline__ void group_prims_from(const T* gexp, int p0, T r2, T r, T (&E)[NP], T (&A)[NP]) {
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
         const T a = ldc(gexp, p0 + p);
         A[p] = a;
         E[p] = (RAD == QMC_RADIAL_GTO_PURE) ? fexp(-a * r2) : fexp(-a * r);
     }
+}
+
+template <typename T, int RAD, int NP>
+__device__ __forceinline__ void group_prims(const QmcParams& prm, int p0, T r2, T r, T (&E)[NP], T (&A)[NP]) {
+    group_prims_from<T, RAD, NP>(group_exponents<T>(prm), p0, r2, r, E, A);
 }
 
 // AO entry i (AO mu, Cartesian powers kx, ky, kz) of a group from its
@@ -318,6 +322,7 @@ __device__ __forceinline__ void ao_row(const Tables& t, const QmcParams& prm, do
 // built from the same atom_geo / group_prims / ao_entry pieces in the same
 // order, so both give the same values.
 struct AoGeneric {
+    static constexpr int kEntries = 0;  // AO entries of the row: known at run time only
     template <typename T, int RAD, bool DERIV, bool TIGHT, typename Sink>
     static __device__ __forceinline__ void row(const Tables& t, const QmcParams& prm, double ex, double ey, double ez,
                                                Sink&& sink) {
@@ -483,20 +488,66 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+constexpr int kBlock = 256;  // threads per block of the wave-per-walker-group kernels
+constexpr int kBlockWaves = kBlock / 64;
+
+// Pivot row (group-relative) of one column for the eliminations of the large
+// buckets: the candidate with the largest |v| (the first one on ties), by one
+// exchange through LDS instead of ceil(log2 n) dependent shuffle rounds.  Each
+// lane's key is the high word of |v|'s FP64 bit pattern (exponent and top 20
+// mantissa bits, monotonic in |v|) with its five low bits replaced by 31 - row,
+// 0 for non-candidates; every lane of the group reads the group's n keys and
+// takes the unsigned maximum.  Magnitudes that differ beyond their top 15
+// mantissa bits are ordered exactly; closer ones count as tied and go to the
+// first row (partial pivoting only needs a large pivot, not the exact
+// maximum).  No candidate (a column past this group's n): row 0.
+// nmax is the wave-uniform bound of n; the reads run to it unmasked (the
+// array's NM-key tail keeps a group ending at lane 63 in bounds) and keys past
+// the lane's own n (other groups' keys) are replaced by 0.
+template <int NM>
+__device__ __forceinline__ int seg_pivot(bool cand, double v, int myrow, int n, int nmax, int gbase) {
+    static_assert(NM <= 32, "five bits of row index");
+    __shared__ uint32_t keys[kBlockWaves * 64 + NM];
+    uint32_t* K = keys + (threadIdx.x >> 6) * 64;
+    const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(fabs(v)) >> 32);
+    const uint32_t key = cand ? ((hi & ~31u) | (uint32_t)(31 - myrow)) : 0u;
+    wave_lds_sync();  // the previous column's key reads are done
+    K[threadIdx.x & 63] = key;
+    wave_lds_sync();
+    uint32_t m = 0;
+    if (__all(n == nmax)) {  // every group has nmax rows (nup = ndown): no masks
+#pragma unroll
+        for (int u = 0; u < NM; ++u)
+            if (u < nmax) m = max(m, K[gbase + u]);
+    } else {
+        // n made opaque per call, so the compiler forms the u < n masks here
+        // rather than holding NM of them in registers over the caller's loops
+        int nn = n;
+        asm volatile("" : "+v"(nn));
+#pragma unroll
+        for (int u = 0; u < NM; ++u) {
+            if (u < nmax) {
+                const uint32_t kv = K[gbase + u];
+                m = max(m, u < nn ? kv : 0u);
+            }
+        }
+    }
+    return m ? 31 - (int)(m & 31u) : 0;
+}
+
 // ------------------------------------------------- determinant-only elimination
 // log|det| and sign of the n x n matrix whose row r lives in lane gbase + r,
-// with the pivot choice and the row arithmetic of lane_gauss_jordan<NM, false>
-// (so the results are identical) but fewer cross-lane moves per column:
+// with the row arithmetic of lane_gauss_jordan<NM, false> but fewer cross-lane
+// moves per column:
 //  * the pivot (largest |a_rk| among the rows not used yet, first one on ties):
-//    up to 8 columns from the n candidates gathered into every lane; for the
-//    larger buckets by a segmented max-reduction over the group's lanes: lane
-//    r merges the best of rows [r + off, r + 2 off) for off = 1, 2, 4, ...,
-//    keeping its own on ties, so lane gbase ends with the group's first
-//    maximum after ceil(log2 n) shuffle rounds instead of n shuffles;
+//    up to 8 columns from the n candidates gathered into every lane, as in
+//    lane_gauss_jordan (identical results); for the larger buckets by
+//    seg_pivot's single LDS exchange of keys;
 //  * the pivot row through rowbuf, this group's NM doubles of LDS: the pivot
 //    lane stores its remaining columns (when may_store), the group reads them
 //    back as broadcast loads (two columns per load) instead of one shuffle
-//    pair per column.
+//    pair per column; for the larger buckets the row includes the pivot itself,
+//    which then needs no shuffle either.
 // Rows already used are never read again (f = 0 for them), so only the rows
 // still to be eliminated are written.
 template <int NM>
@@ -512,14 +563,39 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
     for (int k = 0; k < NM; ++k) {
         if (k < nmax) {
             const bool colk = k < n;
-            int br;
-            if constexpr (NM <= 8) {
+            if constexpr (NM > 8) {
+                const int br = seg_pivot<NM>(!done && colk && myrow < n, a[k], myrow, n, nmax, gbase);
+                const uint64_t bal = __ballot(!done && myrow < br);
+                if (colk) inversions += __popcll(bal & gmask);
+                const bool is_piv = colk && (myrow == br);
+                wave_lds_sync();  // the previous column's reads are done
+                if (is_piv && may_store) {
+#pragma unroll
+                    for (int j = k; j < NM; ++j) rowbuf[j] = a[j];
+                }
+                wave_lds_sync();
+                // past this group's n nobody stores and rowbuf[k] is stale: colk
+                // is false there, so neither the determinant nor a uses it
+                const double piv = rowbuf[k];
+                if (colk) {
+                    singular = singular || (piv == 0.0);
+                    int ex;
+                    mant = frexp(mant * fabs(piv), &ex);
+                    expo += ex;
+                    sgn *= (piv < 0.0) ? -1.0 : 1.0;
+                }
+                const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+                const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
+#pragma unroll
+                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
+                if (is_piv) done = true;
+            } else {
                 const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
                 double c[NM];
 #pragma unroll
                 for (int s = 0; s < NM; ++s) c[s] = __shfl(cand, gbase + (s < n ? s : 0));
                 double bb = -1.0;
-                br = 0;
+                int br = 0;
 #pragma unroll
                 for (int s = 0; s < NM; ++s) {
                     if (s < n && c[s] > bb) {
@@ -527,45 +603,32 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                         br = s;
                     }
                 }
-            } else {
-                double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
-                int brow = myrow;
+                const int plane = gbase + br;
+                const uint64_t bal = __ballot(!done && myrow < br);
+                if (colk) inversions += __popcll(bal & gmask);
+                const double piv = __shfl(a[k], plane);
+                const bool is_piv = colk && (myrow == br);
+                if (colk) {
+                    singular = singular || (piv == 0.0);
+                    int ex;
+                    mant = frexp(mant * fabs(piv), &ex);
+                    expo += ex;
+                    sgn *= (piv < 0.0) ? -1.0 : 1.0;
+                }
+                const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+                const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
+                if (k + 1 < NM) {
+                    wave_lds_sync();  // the previous column's reads are done
+                    if (is_piv && may_store) {
 #pragma unroll
-                for (int off = 1; off < NM; off <<= 1) {
-                    const double ob = __shfl_down(best, off);
-                    const int orow = __shfl_down(brow, off);
-                    if (myrow + off < n && ob > best) {
-                        best = ob;
-                        brow = orow;
+                        for (int j = k + 1; j < NM; ++j) rowbuf[j] = a[j];
                     }
-                }
-                br = __shfl(brow, gbase);
-            }
-            const int plane = gbase + br;
-            const uint64_t bal = __ballot(!done && myrow < br);
-            if (colk) inversions += __popcll(bal & gmask);
-            const double piv = __shfl(a[k], plane);
-            const bool is_piv = colk && (myrow == br);
-            if (colk) {
-                singular = singular || (piv == 0.0);
-                int ex;
-                mant = frexp(mant * fabs(piv), &ex);
-                expo += ex;
-                sgn *= (piv < 0.0) ? -1.0 : 1.0;
-            }
-            const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
-            const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
-            if (k + 1 < NM) {
-                wave_lds_sync();  // the previous column's reads are done
-                if (is_piv && may_store) {
+                    wave_lds_sync();
 #pragma unroll
-                    for (int j = k + 1; j < NM; ++j) rowbuf[j] = a[j];
+                    for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
                 }
-                wave_lds_sync();
-#pragma unroll
-                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
+                if (is_piv) done = true;
             }
-            if (is_piv) done = true;
         }
     }
     if (inversions & 1) sgn = -sgn;
@@ -573,9 +636,10 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
 }
 
 // ------------------------------------------------ augmented elimination (LDS)
-// lane_gauss_jordan<NM, true> with the pivot row passed through LDS: same
-// pivots, same arithmetic, same outputs, for matrices a whose columns
-// n..NM-1 are zero on entry (gathered configurations).
+// lane_gauss_jordan<NM, true> with the pivot row passed through LDS and, above
+// 8 columns, the pivot found by seg_pivot: same arithmetic, same outputs up to
+// the choice between pivots equal to 15 mantissa bits, for matrices a whose
+// columns n..NM-1 are zero on entry (gathered configurations).
 // rowbuf is the group's n * NM doubles (the lane_det_lds buffer).  Up to
 // NM = 16 the pivot lane stores its b[0..NM) at rowbuf[0..NM) and its a[j],
 // k < j < n, at rowbuf[NM - 1 + j] (at most NM + n - 1 doubles, so it fits
@@ -620,18 +684,7 @@ __device__ __forceinline__ void lane_gj_lds(double (&a)[NM], double (&b)[NM], in
                     }
                 }
             } else {
-                double best = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
-                int brow = myrow;
-#pragma unroll
-                for (int off = 1; off < NM; off <<= 1) {
-                    const double ob = __shfl_down(best, off);
-                    const int orow = __shfl_down(brow, off);
-                    if (myrow + off < n && ob > best) {
-                        best = ob;
-                        brow = orow;
-                    }
-                }
-                br = __shfl(brow, gbase);
+                br = seg_pivot<NM>(!done && colk && myrow < n, a[k], myrow, n, nmax, gbase);
             }
             const int plane = gbase + br;
             const uint64_t bal = __ballot(!done && myrow < br);

@@ -16,9 +16,6 @@
 
 namespace qmc {
 
-constexpr int kBlock = 256;
-constexpr int kBlockWaves = kBlock / 64;
-
 // Sum of v over the lanes [base, base+cnt) of this wavefront, in every lane
 // (cnt may differ between groups, cnt_max is its wave-uniform bound): each
 // lane stores v in the wavefront's LDS slot block and adds its group's slots
@@ -126,6 +123,14 @@ struct MoSink {
             W = (const WT*)w;
         }
     }
+    // table pointers of a generated row (exponents, atom centres), made opaque
+    // in the same way per group, so their scalar loads are issued per group
+    // instead of hoisted over the step loop into spilled SGPRs
+    template <typename P>
+    __device__ __forceinline__ const P* pace(const P* p) const {
+        if constexpr (WLDS) asm volatile("" : "+s"(p) : "v"(mo[0]));
+        return p;
+    }
     // row i of the entry-ordered W holds NM columns (zeros past nmo), so the
     // accumulation needs no per-column test
     template <typename V>
@@ -164,12 +169,13 @@ typedef float qmc_floatx16 __attribute__((ext_vector_type(16)));
 template <int NM>
 __device__ __forceinline__ double* wave_rowbuf();
 
-template <int NM, bool WLDS>
+template <int NM, bool WLDS, int NE = 0>
 struct MfmaSink {
     static_assert(NM <= 32, "one 32-column tile");
     const float* W;  // entry-ordered W rows, stride NM (LDS copy with WLDS)
     int nentry;
     int lane;
+    int boff;  // (lane / 32) NM + lane mod 32: this lane's B element of a consecutive pair
     float pend;
     int pend_i;
     bool have;
@@ -179,35 +185,64 @@ struct MfmaSink {
         W = w;
         nentry = n;
         lane = threadIdx.x & 63;
+        boff = (lane >> 5) * NM + (lane & 31);
         have = false;
         pend = 0.0f;
         pend_i = 0;
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0.0f;
     }
-    // generated rows call fence() per exponent group: the W pointer becomes an
-    // opaque value depending on the pending AO value (not on the accumulators,
-    // which would wait for the last MFMA), so B loads are issued where their
-    // group is reached rather than hoisted over the whole step loop
+    // generated rows call fence() per exponent group: the W pointer (in LDS or
+    // global memory) becomes an opaque value depending on the pending AO value
+    // (not on the accumulators, which would wait for the last MFMA), so B loads
+    // and their addresses are formed where their group is reached rather than
+    // hoisted over the whole step loop (where the 60 addresses of a benzene row
+    // were held in spilled VGPRs and reloaded from scratch before every load)
     __device__ __forceinline__ void fence() {
         if constexpr (WLDS) {
             const QMC_LDS float* w = (const QMC_LDS float*)W;
             asm volatile("" : "+s"(w) : "v"(pend));
             W = (const float*)w;
+        } else {
+            asm volatile("" : "+s"(W) : "v"(pend));
         }
     }
-    // entries i0 and i1 (i1 < 0: none) as k = 0 and k = 1 of one MFMA pair
+    template <typename P>
+    __device__ __forceinline__ const P* pace(const P* p) const {
+        asm volatile("" : "+s"(p) : "v"(pend));
+        return p;
+    }
+    __device__ __forceinline__ float wload(int idx) const {
+        if constexpr (WLDS)
+            return ((const QMC_LDS float*)W)[idx];
+        else
+            return ((const __attribute__((address_space(1))) float*)W)[idx];
+    }
+    // entries i0 and i1 (i1 < 0: none, a1 = 0) as k = 0 and k = 1 of one MFMA
+    // pair; lane l supplies B[k][c] = W[i_k][c], k = l / 32, c = l mod 32.  For
+    // consecutive entries (the order every AO row visits them) that is
+    // W[i0 NM + boff]: columns c >= NM then hold values of the next row, which
+    // only reach output columns finish() never reads, and a missing second
+    // entry meets zero A values; reads past the table's end (the last pairs)
+    // are replaced by zero.  The products of the columns kept are those of
+    // B = [W rows i0, i1 | zero columns].
     __device__ __forceinline__ void issue(int i0, int i1, float a0, float a1) {
         const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a0), __float_as_uint(a1), false, false);
         const float A0 = __uint_as_float(r[0]), A1 = __uint_as_float(r[1]);
-        const int col = lane & 31;
-        const int ik = (lane >> 5) ? i1 : i0;
         float b = 0.0f;
-        if (col < NM && ik >= 0 && ik < nentry) {
-            if constexpr (WLDS)
-                b = ((const QMC_LDS float*)W)[ik * NM + col];
-            else
-                b = W[ik * NM + col];
+        if (i1 == i0 + 1 || i1 < 0) {
+            const int idx = i0 * NM + boff;
+            // NE > 0: the row's entry count is a compile-time constant (a
+            // generated row), so this test folds for all but the last pairs
+            const int ne = NE > 0 ? NE : nentry;
+            if ((i0 + 1) * NM + 32 <= ne * NM)  // wave-uniform: every lane in bounds
+                b = wload(idx);
+            else if (idx < ne * NM)
+                b = wload(idx);
+        } else {
+            const int col = lane & 31;
+            const int ik = (lane >> 5) ? i1 : i0;
+            if (col < NM && ik >= 0 && ik < nentry) b = wload(ik * NM + col);
         }
         acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(A0, b, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(A1, b, acc1, 0, 0, 0);
@@ -1148,7 +1183,7 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     // FP32 buckets of 16 and 24 columns contract on the matrix cores (a
     // 64 x nentry x NM product per wavefront); smaller ones in VALU FMAs
     if constexpr (std::is_same<T, float>::value && NM >= 16 && !EXC) {
-        MfmaSink<NM, WLDS> sink;
+        MfmaSink<NM, WLDS, AO::kEntries> sink;
         sink.init((const float*)w, t.nentry);
         AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
         sink.finish(mo);

@@ -68,7 +68,8 @@ def row_source(tab) -> str:
     while g < ngroup:
         at = int(tab.grp_atom[g])
         lines.append("        {  // atom %d" % at)
-        lines.append("            const double dxd = ex - ldc(ac, %d), dyd = ey - ldc(ac, %d), dzd = ez - ldc(ac, %d);"
+        lines.append("            const double* acp = sink.pace(ac);")
+        lines.append("            const double dxd = ex - ldc(acp, %d), dyd = ey - ldc(acp, %d), dzd = ez - ldc(acp, %d);"
                      % (3 * at, 3 * at + 1, 3 * at + 2))
         lines.append("            const AtomGeo<T> G = atom_geo<T, RAD>(dxd, dyd, dzd);")
         while g < ngroup and int(tab.grp_atom[g]) == at:
@@ -80,7 +81,8 @@ def row_source(tab) -> str:
 
             def block(T, geo):
                 out = ["%s E[%d], A[%d];" % (T, npr, npr),
-                       "group_prims<%s, RAD, %d>(prm, %d, %s.r2, %s.r, E, A);" % (T, npr, p0, geo, geo)]
+                       "group_prims_from<%s, RAD, %d>(sink.pace(group_exponents<%s>(prm)), %d, %s.r2, %s.r, E, A);"
+                       % (T, npr, T, p0, geo, geo)]
                 for i in range(i0, i0 + nao):
                     k = int(tab.gao_k[i])
                     kc = ", ".join(repr(float(v)) for v in tab.gao_K[i, :npr])
@@ -123,6 +125,7 @@ namespace qmc {
 namespace {
 
 struct AoSpec {
+    static constexpr int kEntries = %(nentry)d;
     template <typename T, int RAD, bool DERIV, bool TIGHT, typename Sink>
     static __device__ __forceinline__ void row(const Tables& t, const QmcParams& prm, double ex, double ey,
                                                double ez, Sink&& sink) {
