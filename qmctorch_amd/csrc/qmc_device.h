@@ -491,47 +491,46 @@ __device__ __forceinline__ void wave_lds_sync() {
 constexpr int kBlock = 256;  // threads per block of the wave-per-walker-group kernels
 constexpr int kBlockWaves = kBlock / 64;
 
-// Pivot row (group-relative) of one column for the eliminations of the large
-// buckets: the candidate with the largest |v| (the first one on ties), by one
-// exchange through LDS instead of ceil(log2 n) dependent shuffle rounds.  Each
-// lane's key is the high word of |v|'s FP64 bit pattern (exponent and top 20
-// mantissa bits, monotonic in |v|) with its five low bits replaced by 31 - row,
-// 0 for non-candidates; every lane of the group reads the group's n keys and
-// takes the unsigned maximum.  Magnitudes that differ beyond their top 15
-// mantissa bits are ordered exactly; closer ones count as tied and go to the
-// first row (partial pivoting only needs a large pivot, not the exact
-// maximum).  No candidate (a column past this group's n): row 0.
-// nmax is the wave-uniform bound of n; the reads run to it unmasked (the
-// array's NM-key tail keeps a group ending at lane 63 in bounds) and keys past
-// the lane's own n (other groups' keys) are replaced by 0.
+// Pivot row (group-relative) of column k for the eliminations of the large
+// buckets: the candidate with the largest |v| (the first one on ties), by an
+// LDS atomic maximum into a slot of the group instead of ceil(log2 n)
+// dependent shuffle rounds.  Each lane's key is the high word of |v|'s FP64 bit
+// pattern (exponent and top 20 mantissa bits, monotonic in |v|) with its five
+// low bits replaced by 31 - row; non-candidates do not take part.  Magnitudes
+// that differ beyond their top 15 mantissa bits are ordered exactly; closer
+// ones count as tied and go to the first row (partial pivoting only needs a
+// large pivot, not the exact maximum).  No candidate (a column past this
+// group's n): row 0.
+// The slot of a group is its first lane's index; columns alternate between two
+// slot sets, and row 0 of each group clears the other set's slot for column
+// k + 1 while column k's maxima are formed.  The LDS runs one wavefront's
+// instructions in order, so that clear follows the reads of column k - 1 and
+// precedes the maxima of column k + 1.  Lanes outside every walker (real =
+// false, whose group may share its first lane index with a real group) only
+// read.  seg_pivot_init clears set 0 before column 0.
+__device__ __forceinline__ uint32_t* seg_pivot_slots() {
+    __shared__ uint32_t slots[kBlockWaves * 128];
+    return slots + (threadIdx.x >> 6) * 128;
+}
+
+__device__ __forceinline__ void seg_pivot_init(int myrow, int gbase, bool real) {
+    uint32_t* S = seg_pivot_slots();
+    wave_lds_sync();  // earlier eliminations' reads of the slots are done
+    if (real && myrow == 0) S[gbase] = 0u;
+}
+
 template <int NM>
-__device__ __forceinline__ int seg_pivot(bool cand, double v, int myrow, int n, int nmax, int gbase) {
+__device__ __forceinline__ int seg_pivot(int k, bool cand, double v, int myrow, int gbase, bool real) {
     static_assert(NM <= 32, "five bits of row index");
-    __shared__ uint32_t keys[kBlockWaves * 64 + NM];
-    uint32_t* K = keys + (threadIdx.x >> 6) * 64;
+    uint32_t* S = seg_pivot_slots();
     const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(fabs(v)) >> 32);
-    const uint32_t key = cand ? ((hi & ~31u) | (uint32_t)(31 - myrow)) : 0u;
-    wave_lds_sync();  // the previous column's key reads are done
-    K[threadIdx.x & 63] = key;
+    const uint32_t key = (hi & ~31u) | (uint32_t)(31 - myrow);
+    uint32_t* cur = S + (k & 1) * 64 + gbase;
     wave_lds_sync();
-    uint32_t m = 0;
-    if (__all(n == nmax)) {  // every group has nmax rows (nup = ndown): no masks
-#pragma unroll
-        for (int u = 0; u < NM; ++u)
-            if (u < nmax) m = max(m, K[gbase + u]);
-    } else {
-        // n made opaque per call, so the compiler forms the u < n masks here
-        // rather than holding NM of them in registers over the caller's loops
-        int nn = n;
-        asm volatile("" : "+v"(nn));
-#pragma unroll
-        for (int u = 0; u < NM; ++u) {
-            if (u < nmax) {
-                const uint32_t kv = K[gbase + u];
-                m = max(m, u < nn ? kv : 0u);
-            }
-        }
-    }
+    if (real && cand) __hip_atomic_fetch_max(cur, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (real && myrow == 0) S[((k + 1) & 1) * 64 + gbase] = 0u;
+    wave_lds_sync();
+    const uint32_t m = *cur;
     return m ? 31 - (int)(m & 31u) : 0;
 }
 
@@ -551,8 +550,10 @@ __device__ __forceinline__ int seg_pivot(bool cand, double v, int myrow, int n, 
 // Rows already used are never read again (f = 0 for them), so only the rows
 // still to be eliminated are written.
 template <int NM>
-__device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, int myrow, int gbase, uint64_t gmask,
-                                             bool may_store, double* rowbuf, double& logdet, double& sgn) {
+__device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, int myrow, int gbase, int spin,
+                                             uint64_t gmask, bool may_store, double* rowbuf, double& logdet,
+                                             double& sgn) {
+    if constexpr (NM > 8) seg_pivot_init(myrow, gbase, may_store);
     bool done = false;
     sgn = 1.0;
     int inversions = 0;
@@ -564,7 +565,7 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
         if (k < nmax) {
             const bool colk = k < n;
             if constexpr (NM > 8) {
-                const int br = seg_pivot<NM>(!done && colk && myrow < n, a[k], myrow, n, nmax, gbase);
+                const int br = seg_pivot<NM>(k, !done && colk && myrow < n, a[k], myrow, gbase, may_store);
                 const uint64_t bal = __ballot(!done && myrow < br);
                 if (colk) inversions += __popcll(bal & gmask);
                 const bool is_piv = colk && (myrow == br);
@@ -653,8 +654,9 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
 // bounds.
 template <int NM>
 __device__ __forceinline__ void lane_gj_lds(double (&a)[NM], double (&b)[NM], int n, int nmax, int myrow, int gbase,
-                                            uint64_t gmask, bool may_store, double* rowbuf, double& logdet,
+                                            int spin, uint64_t gmask, bool may_store, double* rowbuf, double& logdet,
                                             double& sgn, double& trace_part, int& my_col) {
+    if constexpr (NM > 8) seg_pivot_init(myrow, gbase, may_store);
     bool done = false;
     my_col = -1;
     sgn = 1.0;
@@ -684,7 +686,7 @@ __device__ __forceinline__ void lane_gj_lds(double (&a)[NM], double (&b)[NM], in
                     }
                 }
             } else {
-                br = seg_pivot<NM>(!done && colk && myrow < n, a[k], myrow, n, nmax, gbase);
+                br = seg_pivot<NM>(k, !done && colk && myrow < n, a[k], myrow, gbase, may_store);
             }
             const int plane = gbase + br;
             const uint64_t bal = __ballot(!done && myrow < br);

@@ -437,8 +437,8 @@ __device__ __forceinline__ void lane_gj(const WalkerLane& L, int nmax, double (&
     if constexpr (NM <= 8)
         lane_gauss_jordan<NM, true>(a, b, L.n_sig, nmax, L.myrow, L.gbase, L.gmask, lg, sg, tp, col);
     else
-        lane_gj_lds<NM>(a, b, L.n_sig, nmax, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg, tp,
-                        col);
+        lane_gj_lds<NM>(a, b, L.n_sig, nmax, L.myrow, L.gbase, L.spin, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg,
+                        tp, col);
 }
 
 // Single configuration, single CI term: log|D_up D_dn| and its sign (ground state).
@@ -449,7 +449,7 @@ __device__ __forceinline__ void slater_single(const Tables& t, const QmcParams& 
 #pragma unroll
     for (int j = 0; j < NM; ++j) a[j] = (double)mo[j];
     double lg, sg;
-    lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg);
+    lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.spin, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg, sg);
     const int dn_lane = L.wbase + (t.ndown > 0 ? t.nup : 0);
     const double lu = __shfl(lg, L.wbase), su = __shfl(sg, L.wbase);
     const double ldn = __shfl(lg, dn_lane), sdn = __shfl(sg, dn_lane);
@@ -506,8 +506,8 @@ __device__ __forceinline__ void slater_combine(const Tables& t, const QmcParams&
         if (KIN) gather_conf<NM>(t, L, c, bk, b);
         double lg, sg, tp = 0.0;
         if constexpr (!KIN) {
-            lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg,
-                             sg);
+            lane_det_lds<NM>(a, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.spin, L.gmask, L.lane_real,
+                             det_rowbuf<NM>(L), lg, sg);
         } else {
             int col;
             lane_gj<NM>(L, t.nsig_max, a, b, lg, sg, tp, col);
@@ -1828,8 +1828,8 @@ __device__ __forceinline__ void conf_dets(const Tables& t, const WalkerLane& L, 
         if constexpr (INV)
             lane_gj<NM>(L, t.nsig_max, aa, bb, lg, sg, tp, col);
         else
-            lane_det_lds<NM>(aa, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.gmask, L.lane_real, det_rowbuf<NM>(L), lg,
-                             sg);
+            lane_det_lds<NM>(aa, L.n_sig, t.nsig_max, L.myrow, L.gbase, L.spin, L.gmask, L.lane_real,
+                             det_rowbuf<NM>(L), lg, sg);
 #pragma unroll
         for (int c2 = 0; c2 < QMC_MAXCONF; ++c2)
             if (c2 == c) {
