@@ -507,10 +507,18 @@ constexpr int kBlockWaves = kBlock / 64;
 // instructions in order, so that clear follows the reads of column k - 1 and
 // precedes the maxima of column k + 1.  Lanes outside every walker (real =
 // false, whose group may share its first lane index with a real group) only
-// read.  seg_pivot_init clears set 0 before column 0.
+// read.  seg_pivot_init clears set 0 before column 0.  The wait after the
+// maxima also completes the caller's LDS stores issued before the call.
 __device__ __forceinline__ uint32_t* seg_pivot_slots() {
     __shared__ uint32_t slots[kBlockWaves * 128];
     return slots + (threadIdx.x >> 6) * 128;
+}
+
+// Reciprocal of each group's published pivot (lane_det_lds), at the group's
+// first lane index.
+__device__ __forceinline__ double* det_recips() {
+    __shared__ double r[kBlockWaves * 64];
+    return r + (threadIdx.x >> 6) * 64;
 }
 
 __device__ __forceinline__ void seg_pivot_init(int myrow, int gbase, bool real) {
@@ -519,34 +527,46 @@ __device__ __forceinline__ void seg_pivot_init(int myrow, int gbase, bool real) 
     if (real && myrow == 0) S[gbase] = 0u;
 }
 
+// seg_pivot_post issues column k's maximum (and the next column's clear);
+// seg_pivot_get waits for it, and for every LDS store the caller issued in
+// between, and returns the pivot row.  seg_pivot does both.
 template <int NM>
-__device__ __forceinline__ int seg_pivot(int k, bool cand, double v, int myrow, int gbase, bool real) {
+__device__ __forceinline__ void seg_pivot_post(int k, bool cand, double v, int myrow, int gbase, bool real) {
     static_assert(NM <= 32, "five bits of row index");
     uint32_t* S = seg_pivot_slots();
     const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(fabs(v)) >> 32);
     const uint32_t key = (hi & ~31u) | (uint32_t)(31 - myrow);
-    uint32_t* cur = S + (k & 1) * 64 + gbase;
-    wave_lds_sync();
-    if (real && cand) __hip_atomic_fetch_max(cur, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (real && cand)
+        __hip_atomic_fetch_max(S + (k & 1) * 64 + gbase, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (real && myrow == 0) S[((k + 1) & 1) * 64 + gbase] = 0u;
+}
+
+__device__ __forceinline__ int seg_pivot_get(int k, int gbase) {
     wave_lds_sync();
-    const uint32_t m = *cur;
+    const uint32_t m = seg_pivot_slots()[(k & 1) * 64 + gbase];
     return m ? 31 - (int)(m & 31u) : 0;
+}
+
+template <int NM>
+__device__ __forceinline__ int seg_pivot(int k, bool cand, double v, int myrow, int gbase, bool real) {
+    seg_pivot_post<NM>(k, cand, v, myrow, gbase, real);
+    return seg_pivot_get(k, gbase);
 }
 
 // ------------------------------------------------- determinant-only elimination
 // log|det| and sign of the n x n matrix whose row r lives in lane gbase + r,
 // with the row arithmetic of lane_gauss_jordan<NM, false> but fewer cross-lane
 // moves per column:
-//  * the pivot (largest |a_rk| among the rows not used yet, first one on ties):
-//    up to 8 columns from the n candidates gathered into every lane, as in
-//    lane_gauss_jordan (identical results); for the larger buckets by
-//    seg_pivot's single LDS exchange of keys;
+//  * the pivot (largest |a_rk| among the rows not used yet, first one on ties,
+//    compared through seg_pivot's 32-bit keys): up to 8 columns from the n
+//    keys gathered into every lane by shuffles; for the larger buckets by
+//    seg_pivot's LDS atomic maximum;
 //  * the pivot row through rowbuf, this group's NM doubles of LDS: the pivot
-//    lane stores its remaining columns (when may_store), the group reads them
-//    back as broadcast loads (two columns per load) instead of one shuffle
-//    pair per column; for the larger buckets the row includes the pivot itself,
-//    which then needs no shuffle either.
+//    lane stores its remaining columns, the pivot itself included (when
+//    may_store), and, in det_recips(), the pivot's reciprocal, which every
+//    candidate forms while the pivot is searched (the division leaves the
+//    column's critical path); the group reads them back as broadcast loads
+//    (two columns per load) instead of one shuffle pair per column.
 // Rows already used are never read again (f = 0 for them), so only the rows
 // still to be eliminated are written.
 template <int NM>
@@ -554,6 +574,7 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                                              uint64_t gmask, bool may_store, double* rowbuf, double& logdet,
                                              double& sgn) {
     if constexpr (NM > 8) seg_pivot_init(myrow, gbase, may_store);
+    double* const recip = det_recips() + gbase;  // the group's pivot reciprocal
     bool done = false;
     sgn = 1.0;
     int inversions = 0;
@@ -565,7 +586,15 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
         if (k < nmax) {
             const bool colk = k < n;
             if constexpr (NM > 8) {
-                const int br = seg_pivot<NM>(k, !done && colk && myrow < n, a[k], myrow, gbase, may_store);
+                // the reciprocal of every candidate's pivot is formed while the
+                // search runs; the pivot lane then publishes its remaining
+                // columns and that reciprocal (rowbuf[0..NM) and the group's
+                // det_recips() entry), which the group reads back at once
+                const bool cand = !done && colk && myrow < n;
+                seg_pivot_post<NM>(k, cand, a[k], myrow, gbase, may_store);
+                double rk = (a[k] != 0.0) ? 1.0 / a[k] : 0.0;
+                asm volatile("" : "+v"(rk));  // formed here, not sunk into the pivot lane's store
+                const int br = seg_pivot_get(k, gbase);
                 const uint64_t bal = __ballot(!done && myrow < br);
                 if (colk) inversions += __popcll(bal & gmask);
                 const bool is_piv = colk && (myrow == br);
@@ -573,61 +602,60 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                 if (is_piv && may_store) {
 #pragma unroll
                     for (int j = k; j < NM; ++j) rowbuf[j] = a[j];
+                    *recip = rk;
                 }
                 wave_lds_sync();
-                // past this group's n nobody stores and rowbuf[k] is stale: colk
+                // past this group's n nobody stores and the row is stale: colk
                 // is false there, so neither the determinant nor a uses it
                 const double piv = rowbuf[k];
-                if (colk) {
-                    singular = singular || (piv == 0.0);
+                const double ip = *recip;
+                {
+                    const double pv = colk ? piv : 1.0;
+                    singular = singular || (pv == 0.0);
                     int ex;
-                    mant = frexp(mant * fabs(piv), &ex);
+                    mant = frexp(mant * fabs(pv), &ex);
                     expo += ex;
-                    sgn *= (piv < 0.0) ? -1.0 : 1.0;
+                    sgn *= (pv < 0.0) ? -1.0 : 1.0;
                 }
-                const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
                 const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
 #pragma unroll
                 for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
                 if (is_piv) done = true;
             } else {
-                const double cand = (!done && colk && myrow < n) ? fabs(a[k]) : -1.0;
-                double c[NM];
+                // keys as in seg_pivot, gathered from the group's n lanes (reads
+                // past the last row repeat it, which leaves the maximum alone)
+                const bool cand = !done && colk && myrow < n;
+                const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(fabs(a[k])) >> 32);
+                const uint32_t key = cand ? ((hi & ~31u) | (uint32_t)(31 - myrow)) : 0u;
+                double rk = (a[k] != 0.0) ? 1.0 / a[k] : 0.0;
+                asm volatile("" : "+v"(rk));  // formed here, not sunk into the pivot lane's store
+                uint32_t m = 0;
 #pragma unroll
-                for (int s = 0; s < NM; ++s) c[s] = __shfl(cand, gbase + (s < n ? s : 0));
-                double bb = -1.0;
-                int br = 0;
-#pragma unroll
-                for (int s = 0; s < NM; ++s) {
-                    if (s < n && c[s] > bb) {
-                        bb = c[s];
-                        br = s;
-                    }
-                }
-                const int plane = gbase + br;
+                for (int s = 0; s < NM; ++s) m = max(m, (uint32_t)__shfl((int)key, gbase + (s < n ? s : n - 1)));
+                const int br = m ? 31 - (int)(m & 31u) : 0;
                 const uint64_t bal = __ballot(!done && myrow < br);
                 if (colk) inversions += __popcll(bal & gmask);
-                const double piv = __shfl(a[k], plane);
                 const bool is_piv = colk && (myrow == br);
-                if (colk) {
-                    singular = singular || (piv == 0.0);
+                wave_lds_sync();  // the previous column's reads are done
+                if (is_piv && may_store) {
+#pragma unroll
+                    for (int j = k; j < NM; ++j) rowbuf[j] = a[j];
+                    *recip = rk;
+                }
+                wave_lds_sync();
+                const double piv = rowbuf[k];
+                const double ip = *recip;
+                {
+                    const double pv = colk ? piv : 1.0;
+                    singular = singular || (pv == 0.0);
                     int ex;
-                    mant = frexp(mant * fabs(piv), &ex);
+                    mant = frexp(mant * fabs(pv), &ex);
                     expo += ex;
-                    sgn *= (piv < 0.0) ? -1.0 : 1.0;
+                    sgn *= (pv < 0.0) ? -1.0 : 1.0;
                 }
-                const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
                 const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
-                if (k + 1 < NM) {
-                    wave_lds_sync();  // the previous column's reads are done
-                    if (is_piv && may_store) {
 #pragma unroll
-                        for (int j = k + 1; j < NM; ++j) rowbuf[j] = a[j];
-                    }
-                    wave_lds_sync();
-#pragma unroll
-                    for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
-                }
+                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
                 if (is_piv) done = true;
             }
         }
