@@ -561,12 +561,14 @@ __device__ __forceinline__ int seg_pivot(int k, bool cand, double v, int myrow, 
 //    compared through seg_pivot's 32-bit keys): up to 8 columns from the n
 //    keys gathered into every lane by shuffles; for the larger buckets by
 //    seg_pivot's LDS atomic maximum;
-//  * the pivot row through rowbuf, this group's NM doubles of LDS: the pivot
-//    lane stores its remaining columns, the pivot itself included (when
-//    may_store), and, in det_recips(), the pivot's reciprocal, which every
-//    candidate forms while the pivot is searched (the division leaves the
-//    column's critical path); the group reads them back as broadcast loads
-//    (two columns per load) instead of one shuffle pair per column.
+//  * the pivot row through LDS, read back as broadcast loads (two columns per
+//    load) instead of one shuffle pair per column, with the pivot itself and
+//    its reciprocal (det_recips()), which every candidate forms while the
+//    pivot is searched (the division leaves the column's critical path): for
+//    the larger buckets the pivot lane stores its remaining columns at
+//    rowbuf[0..NM) once the pivot is known; up to 8 columns every candidate
+//    row r stores them at rowbuf[r NM ..] during the search (n NM doubles).
+// Stores happen only when may_store.
 // Rows already used are never read again (f = 0 for them), so only the rows
 // still to be eliminated are written.
 template <int NM>
@@ -623,12 +625,21 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                 if (is_piv) done = true;
             } else {
                 // keys as in seg_pivot, gathered from the group's n lanes (reads
-                // past the last row repeat it, which leaves the maximum alone)
+                // past the last row repeat it, which leaves the maximum alone);
+                // meanwhile every candidate row r publishes its remaining
+                // columns at rowbuf[r NM + k ..] and its pivot's reciprocal at
+                // recip[r], and the group then reads the chosen row back (slots
+                // of rows already used are not written again, so rows read in
+                // earlier columns stay intact)
                 const bool cand = !done && colk && myrow < n;
                 const uint32_t hi = (uint32_t)((uint64_t)__double_as_longlong(fabs(a[k])) >> 32);
                 const uint32_t key = cand ? ((hi & ~31u) | (uint32_t)(31 - myrow)) : 0u;
-                double rk = (a[k] != 0.0) ? 1.0 / a[k] : 0.0;
-                asm volatile("" : "+v"(rk));  // formed here, not sunk into the pivot lane's store
+                if (cand && may_store) {
+                    double* const slot = rowbuf + myrow * NM;
+#pragma unroll
+                    for (int j = k; j < NM; ++j) slot[j] = a[j];
+                    recip[myrow] = (a[k] != 0.0) ? 1.0 / a[k] : 0.0;
+                }
                 uint32_t m = 0;
 #pragma unroll
                 for (int s = 0; s < NM; ++s) m = max(m, (uint32_t)__shfl((int)key, gbase + (s < n ? s : n - 1)));
@@ -636,15 +647,10 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                 const uint64_t bal = __ballot(!done && myrow < br);
                 if (colk) inversions += __popcll(bal & gmask);
                 const bool is_piv = colk && (myrow == br);
-                wave_lds_sync();  // the previous column's reads are done
-                if (is_piv && may_store) {
-#pragma unroll
-                    for (int j = k; j < NM; ++j) rowbuf[j] = a[j];
-                    *recip = rk;
-                }
-                wave_lds_sync();
-                const double piv = rowbuf[k];
-                const double ip = *recip;
+                wave_lds_sync();  // the candidates' stores are done
+                const double* const prow = rowbuf + br * NM;
+                const double piv = prow[k];
+                const double ip = recip[br];
                 {
                     const double pv = colk ? piv : 1.0;
                     singular = singular || (pv == 0.0);
@@ -655,7 +661,7 @@ __device__ __forceinline__ void lane_det_lds(double (&a)[NM], int n, int nmax, i
                 }
                 const double f = (colk && !is_piv && !done) ? a[k] * ip : 0.0;
 #pragma unroll
-                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * rowbuf[j];
+                for (int j = k + 1; j < NM; ++j) a[j] = a[j] - f * prow[j];
                 if (is_piv) done = true;
             }
         }
