@@ -355,7 +355,7 @@ int qmc_plan_create(const QmcSystemDesc* d, QmcPlan** out) {
             t.exc_dn = (decltype(t.exc_dn))p->upload(exc_dn.data(), exc_dn.size());
         }
         // generated kernels have no excitation path
-        p->spec = exc ? nullptr : find_spec(layout_signature(d), nm_bucket_fine(t));
+        p->spec = exc ? nullptr : find_spec(layout_signature(d), nm_bucket_fine(t), (64 / t.nelec) * t.nelec);
     } catch (const std::exception& ex) {
         delete p;
         return fail(QMC_ERR_HIP, ex.what());

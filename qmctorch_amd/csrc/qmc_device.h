@@ -322,7 +322,8 @@ __device__ __forceinline__ void ao_row(const Tables& t, const QmcParams& prm, do
 // built from the same atom_geo / group_prims / ao_entry pieces in the same
 // order, so both give the same values.
 struct AoGeneric {
-    static constexpr int kEntries = 0;  // AO entries of the row: known at run time only
+    static constexpr int kEntries = 0;    // AO entries of the row: known at run time only
+    static constexpr int kRowBlocks = 4;  // 16-lane row blocks of the MFMA contraction: all
     template <typename T, int RAD, bool DERIV, bool TIGHT, typename Sink>
     static __device__ __forceinline__ void row(const Tables& t, const QmcParams& prm, double ex, double ey, double ez,
                                                Sink&& sink) {

@@ -154,9 +154,12 @@ struct MoSink {
 // AO -> MO contraction on the matrix cores for the large register buckets
 // (molecular_orbitals.py:79-95): MO = AO W with the wavefront's 64 lanes as the
 // rows of AO (one electron each), the AO entries as K and the bucket's columns
-// as N, by v_mfma_f32_32x32x2f32 (FP32 products, FP32 accumulation) in two
-// 32-row tiles.  AO entries arrive one at a time and are issued in pairs: for
-// entries (i, j) one v_permlane32_swap turns the lanes' values a_i, a_j
+// as N.  Two forms: MfmaPairSink for the table-driven row (whose entries and
+// pairing are run-time values), MfmaQuadSink for generated rows.
+//
+// MfmaPairSink: v_mfma_f32_32x32x2f32 (FP32 products, FP32 accumulation) in
+// two 32-row tiles.  AO entries arrive one at a time and are issued in pairs:
+// for entries (i, j) one v_permlane32_swap turns the lanes' values a_i, a_j
 // into the A operands of the two tiles (lanes 0-31 hold k = 0 = entry i,
 // lanes 32-63 k = 1 = entry j, of rows 0-31 and of rows 32-63), and lane
 // c + 32 k loads B[k][c] = W[k ? j : i][c].  finish() moves the accumulators
@@ -165,12 +168,13 @@ struct MoSink {
 // Slater elimination uses next; 64 rows of NM + 1 floats fit in its 64 NM
 // doubles).
 typedef float qmc_floatx16 __attribute__((ext_vector_type(16)));
+typedef float qmc_floatx4 __attribute__((ext_vector_type(4)));
 
 template <int NM>
 __device__ __forceinline__ double* wave_rowbuf();
 
 template <int NM, bool WLDS, int NE = 0>
-struct MfmaSink {
+struct MfmaPairSink {
     static_assert(NM <= 32, "one 32-column tile");
     const float* W;  // entry-ordered W rows, stride NM (LDS copy with WLDS)
     int nentry;
@@ -276,6 +280,150 @@ struct MfmaSink {
         wave_lds_sync();
 #pragma unroll
         for (int m = 0; m < NM; ++m) mo[m] = T[lane * S + m];
+        wave_lds_sync();  // the tile is read before the elimination stores rows
+    }
+};
+
+// MfmaQuadSink: v_mfma_f32_16x16x4f32 (FP32 products, FP32 accumulation) on
+// 16-row by 16-column tiles, so that only the RB row blocks holding real
+// walkers' electrons (a generated row for one benzene walker per wavefront, 42
+// lanes: 3 of 4) and the column blocks of the bucket (1 for 16, 2 for 24) are
+// computed.  AO entries arrive one at a time and are issued in fours: lane
+// 16 g + r holds the values v_0..v_3 of
+// entries i..i+3 for electron 16 g + r, and two v_permlane32_swap and two
+// v_permlane16_swap transpose them so that lane r + 16 k holds v_k of electron
+// 16 b + r in register T_b, the A operand of row block b (A[r][k] in lane
+// r + 16 k).  Lane j + 16 k loads B[k][j] = W[i + k][16 c + j] for column
+// block c.  finish() moves the accumulators (lane l holds column l mod 16 of
+// rows 4 (l / 16) + r, r < 4, of its tile) into each lane's MO row through the
+// wave's row buffer (the LDS the Slater elimination uses next; 64 rows of
+// NM + 1 floats fit in its 64 NM doubles).
+template <int NM, bool WLDS, int NE = 0, int RB = 4>
+struct MfmaQuadSink {
+    static_assert(NM <= 32, "two 16-column blocks");
+    static_assert(RB >= 1 && RB <= 4, "16-lane row blocks of a wavefront");
+    static constexpr int NCB = (NM + 15) / 16;  // 16-column blocks
+    const float* W;  // entry-ordered W rows, stride NM (LDS copy with WLDS)
+    int nentry;
+    int lane;
+    int boff;    // (lane / 16) NM + lane mod 16: this lane's B element of four consecutive entries
+    int npend;   // entries pending, the first one pend_i
+    int pend_i;
+    float p0, p1, p2, p3;  // pending values, the newest in p3 (shifted in)
+    qmc_floatx4 acc[RB][NCB];
+
+    __device__ __forceinline__ void init(const float* w, int n) {
+        W = w;
+        nentry = n;
+        lane = threadIdx.x & 63;
+        boff = (lane >> 4) * NM + (lane & 15);
+        npend = 0;
+        pend_i = 0;
+        p0 = p1 = p2 = p3 = 0.0f;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) acc[b][c] = qmc_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    // generated rows call fence() per exponent group: the W pointer (in LDS or
+    // global memory) becomes an opaque value depending on the newest AO value
+    // (not on the accumulators, which would wait for the last MFMA), so B loads
+    // and their addresses are formed where their group is reached rather than
+    // hoisted over the whole step loop (where a benzene row's 60 addresses were
+    // held in spilled VGPRs and reloaded from scratch before every load)
+    __device__ __forceinline__ void fence() {
+        if constexpr (WLDS) {
+            const QMC_LDS float* w = (const QMC_LDS float*)W;
+            asm volatile("" : "+s"(w) : "v"(p3));
+            W = (const float*)w;
+        } else {
+            asm volatile("" : "+s"(W) : "v"(p3));
+        }
+    }
+    template <typename P>
+    __device__ __forceinline__ const P* pace(const P* p) const {
+        asm volatile("" : "+s"(p) : "v"(p3));
+        return p;
+    }
+    __device__ __forceinline__ float wload(int idx) const {
+        if constexpr (WLDS)
+            return ((const QMC_LDS float*)W)[idx];
+        else
+            return ((const __attribute__((address_space(1))) float*)W)[idx];
+    }
+    // the four pending entries pend_i.. (missing ones zero) as K of one MFMA per
+    // tile.  B[k][j] = W[pend_i + k][16 c + j] is W[pend_i NM + boff + 16 c]:
+    // columns past NM then hold values of the next row, which only reach
+    // output columns finish() never reads, and a missing entry meets zero A
+    // values; reads past the table's end (the last quadruples) are replaced by
+    // zero.
+    __device__ __forceinline__ void issue() {
+        const auto r02 = __builtin_amdgcn_permlane32_swap(__float_as_uint(p0), __float_as_uint(p2), false, false);
+        const auto r13 = __builtin_amdgcn_permlane32_swap(__float_as_uint(p1), __float_as_uint(p3), false, false);
+        const auto s01 = __builtin_amdgcn_permlane16_swap(r02[0], r13[0], false, false);
+        const auto s23 = __builtin_amdgcn_permlane16_swap(r02[1], r13[1], false, false);
+        const float T[4] = {__uint_as_float(s01[0]), __uint_as_float(s01[1]), __uint_as_float(s23[0]),
+                            __uint_as_float(s23[1])};
+        // NE > 0: the row's entry count is a compile-time constant (a generated
+        // row), so this test folds for all but the last quadruples
+        const int ne = NE > 0 ? NE : nentry;
+#pragma unroll
+        for (int c = 0; c < NCB; ++c) {
+            const int idx = pend_i * NM + boff + 16 * c;
+            float b = 0.0f;
+            if ((pend_i + 3) * NM + 16 * c + 16 <= ne * NM)  // wave-uniform: every lane in bounds
+                b = wload(idx);
+            else if (idx < ne * NM)
+                b = wload(idx);
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(T[rb], b, acc[rb][c], 0, 0, 0);
+        }
+        npend = 0;
+        p0 = p1 = p2 = p3 = 0.0f;
+    }
+    // shift in zeros until the pending entries fill k = 0..npend-1, then issue
+    __device__ __forceinline__ void flush() {
+        if (npend == 0) return;
+        while (npend < 4) {
+            p0 = p1;
+            p1 = p2;
+            p2 = p3;
+            p3 = 0.0f;
+            ++npend;
+        }
+        issue();
+    }
+    template <typename V>
+    __device__ __forceinline__ void operator()(int i, int, V ao, V, V, V, V) {
+        if (npend > 0 && i != pend_i + npend) flush();  // entries arrive in order; else start a new four
+        if (npend == 0) pend_i = i;
+        p0 = p1;
+        p1 = p2;
+        p2 = p3;
+        p3 = (float)ao;
+        if (++npend == 4) issue();
+    }
+    __device__ __forceinline__ void finish(float (&mo)[NM]) {
+        flush();
+        constexpr int S = NM + 1;  // odd row stride: conflict-free column reads
+        float* Tt = (float*)wave_rowbuf<NM>();
+        wave_lds_sync();  // the previous elimination's reads of the buffer are done
+        const int col = lane & 15, rq = 4 * (lane >> 4);
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+#pragma unroll
+            for (int c = 0; c < NCB; ++c) {
+                if (16 * c + col < NM) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Tt[(16 * rb + rq + r) * S + 16 * c + col] = acc[rb][c][r];
+                }
+            }
+        }
+        wave_lds_sync();
+        // lanes past the computed row blocks (outside every walker) take zeros
+        const bool have = lane < 16 * RB;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) mo[m] = have ? Tt[lane * S + m] : 0.0f;
         wave_lds_sync();  // the tile is read before the elimination stores rows
     }
 };
@@ -1183,7 +1331,9 @@ __device__ __forceinline__ double eval_logpsi(const Tables& t, const QmcParams& 
     // FP32 buckets of 16 and 24 columns contract on the matrix cores (a
     // 64 x nentry x NM product per wavefront); smaller ones in VALU FMAs
     if constexpr (std::is_same<T, float>::value && NM >= 16 && !EXC) {
-        MfmaSink<NM, WLDS, AO::kEntries> sink;
+        using Sink = typename std::conditional<(AO::kEntries > 0), MfmaQuadSink<NM, WLDS, AO::kEntries, AO::kRowBlocks>,
+                                               MfmaPairSink<NM, WLDS>>::type;
+        Sink sink;
         sink.init((const float*)w, t.nentry);
         AO::template row<T, RAD, false, false>(t, prm, x, y, z, sink);
         sink.finish(mo);

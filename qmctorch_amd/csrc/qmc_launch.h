@@ -95,6 +95,7 @@ inline dim3 exc_grid(const Tables& t, int64_t nw) {
 struct SpecEntry {
     uint64_t signature;  // layout_signature of the layout
     int nm;              // register bucket the kernels were built for
+    int max_rows;        // lanes of real walkers per wavefront (G Ne) the sweeps' matrix tiles cover
     const char* name;
     int (*sweep)(const SArgs& a, int precision, hipStream_t s);
     int (*sweep_batch)(const SJob* jobs, int njobs, int nwaves, int precision, hipStream_t s);
@@ -103,8 +104,9 @@ struct SpecEntry {
     int (*hmc)(const HArgs& a, int precision, hipStream_t s);  // mixed HMC / drift
 };
 
-// the generated layout with this signature and bucket, or nullptr
-const SpecEntry* find_spec(uint64_t signature, int nm);
+// the generated layout with this signature and bucket whose kernels cover
+// `rows` lanes of real walkers per wavefront, or nullptr
+const SpecEntry* find_spec(uint64_t signature, int nm, int rows);
 
 #define QMC_BY_RAD(fn, rad, ...) ((rad) == QMC_RADIAL_STO ? fn<QMC_RADIAL_STO>(__VA_ARGS__) : fn<QMC_RADIAL_GTO_PURE>(__VA_ARGS__))
 

@@ -50,6 +50,14 @@ def layout_signature(tab) -> int:
     return h
 
 
+def row_blocks(tab) -> int:
+    """16-lane row blocks of a wavefront holding the walkers of ``tab``'s
+    molecule (one lane per electron, 64 // Ne walkers per wavefront): the rows
+    the generated sweeps' matrix tiles compute."""
+    ne = int(tab.nelec)
+    return (64 // ne * ne + 15) // 16
+
+
 def nm_bucket_fine(nmo: int, nup: int, ndown: int) -> int:
     need = max(nmo, nup, ndown, 1)
     if need <= 8:
@@ -126,6 +134,7 @@ namespace {
 
 struct AoSpec {
     static constexpr int kEntries = %(nentry)d;
+    static constexpr int kRowBlocks = %(rowblocks)d;  // 16-lane row blocks holding the catalog molecule's walkers
     template <typename T, int RAD, bool DERIV, bool TIGHT, typename Sink>
     static __device__ __forceinline__ void row(const Tables& t, const QmcParams& prm, double ex, double ey,
                                                double ez, Sink&& sink) {
@@ -192,11 +201,11 @@ namespace qmc {
 %(decls)s
 // host-side table (the device pass of this unit needs none of it)
 #if !defined(__HIP_DEVICE_COMPILE__)
-const SpecEntry* find_spec(uint64_t signature, int nm) {
+const SpecEntry* find_spec(uint64_t signature, int nm, int rows) {
     static const SpecEntry table[] = {
-%(rows)s        {0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
+%(rows)s        {0, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr}};
     for (const SpecEntry* e = table; e->name; ++e)
-        if (e->signature == signature && e->nm == nm) return e;
+        if (e->signature == signature && e->nm == nm && rows <= e->max_rows) return e;
     return nullptr;
 }
 #endif
@@ -232,7 +241,7 @@ def unit_sources(name: str, tab, nm: int, mc: int = 1) -> Dict[str, str]:
     in parallel.  mc bounds the unique configurations per spin of the HMC
     instance (1, or QMC_MAXCONF)."""
     head = ROW % dict(name=name, ngroup=len(tab.grp_atom), nentry=len(tab.gao_mu), nm=nm, rad=int(tab.radial),
-                      body=row_source(tab))
+                      rowblocks=row_blocks(tab), body=row_source(tab))
     single, batch = sweep_launch(len(tab.gao_mu), nm)
     sweep = SWEEP % dict(name=name, launch=single, launch_batch=batch)
     return {"qmc_spec_%s_sweep.hip" % name: head + sweep,
@@ -248,9 +257,9 @@ def registry_source(layouts) -> str:
                   "int spec_%(n)s_wf(const KArgs&, int, int, hipStream_t);\n"
                   "int spec_%(n)s_wf_batch(const WJob*, int, int, int, int, hipStream_t);\n"
                   "int spec_%(n)s_hmc(const HArgs&, int, hipStream_t);\n" % dict(n=name))
-        rows += ("        {0x%016xull, %d, \"%s\", &spec_%s_sweep, &spec_%s_sweep_batch, &spec_%s_wf, "
-                 "&spec_%s_wf_batch, &spec_%s_hmc},\n" % (layout_signature(tab), nm, name, name, name, name, name,
-                                                           name))
+        rows += ("        {0x%016xull, %d, %d, \"%s\", &spec_%s_sweep, &spec_%s_sweep_batch, &spec_%s_wf, "
+                 "&spec_%s_wf_batch, &spec_%s_hmc},\n" % (layout_signature(tab), nm, 16 * row_blocks(tab), name,
+                                                           name, name, name, name, name))
     return REGISTRY % dict(decls=decls, rows=rows)
 
 
