@@ -37,9 +37,9 @@ enum { QMC_PREC_FP64 = 0, QMC_PREC_MIXED = 1 };
 #define QMC_MAXCONF 8
 /* Plans with more unique configurations per spin (up to QMC_MAXEXC) must be
  * single and double excitations of the ground state (columns 0..n-1); psi, pdf,
- * the local energy, the Metropolis sweep, the drift, HMC and the adjoints
- * (qmc_logpsi_grad, qmc_logpsi_grad_noalloc, qmc_eloc_grad) then use the
- * excitation (trace-trick) kernels; the row kernels and the batched launches
+ * the local energy, the Metropolis sweep, the drift, HMC, the adjoints
+ * (qmc_logpsi_grad, qmc_logpsi_grad_noalloc, qmc_eloc_grad) and the batched
+ * launches then use the excitation (trace-trick) kernels; the row kernels
  * return QMC_ERR_UNSUPPORTED. */
 #define QMC_MAXEXC 1024
 

@@ -212,7 +212,8 @@ int refuse_excitations(const QmcPlan* plan) {
     if (plan && plan->t.exc_mode)
         return fail(QMC_ERR_UNSUPPORTED, "more than 8 unique configurations per spin: this entry point has no "
                                          "excitation (trace-trick) path (psi, pdf, local energy, the Metropolis "
-                                         "sweep, the drift, HMC and the log|psi| and local-energy adjoints have)");
+                                         "sweep, the drift, HMC, the log|psi| and local-energy adjoints and the "
+                                         "batched launches have)");
     return QMC_OK;
 }
 
@@ -732,27 +733,32 @@ int take_slot(QmcBatch* b, int& slot) {
 
 struct Group {
     int rad, bucket;
+    bool exc;  // excitation plans: their own kernels, one wavefront per block
     const SpecEntry* spec;
     std::vector<int> idx;
+    size_t lds;  // exc: the largest exc_lds_bytes of the group's jobs
 };
 
-// Groups the jobs by (radial type, register bucket), costliest systems first
-// inside a group.  Returns an error for jobs without a bucket.
+// Groups the jobs by (radial type, register bucket, excitation path, generated
+// layout), costliest systems first inside a group.  Returns an error for jobs
+// without a bucket.
 template <typename JobT>
 int group_jobs(const JobT* jobs, int njobs, std::vector<Group>& groups) {
     for (int j = 0; j < njobs; ++j) {
         const Tables& t = jobs[j].plan->t;
         const int bk = nm_bucket_fine(t);
         if (bk < 0) return fail(QMC_ERR_UNSUPPORTED, kNoBucket);
-        const SpecEntry* sp = jobs[j].plan->use_spec ? jobs[j].plan->spec : nullptr;
+        const bool exc = t.exc_mode != 0;
+        const SpecEntry* sp = (!exc && jobs[j].plan->use_spec) ? jobs[j].plan->spec : nullptr;
         Group* g = nullptr;
         for (auto& x : groups)
-            if (x.rad == t.radial && x.bucket == bk && x.spec == sp) g = &x;
+            if (x.rad == t.radial && x.bucket == bk && x.exc == exc && x.spec == sp) g = &x;
         if (!g) {
-            groups.push_back(Group{t.radial, bk, sp, {}});
+            groups.push_back(Group{t.radial, bk, exc, sp, {}, 0});
             g = &groups.back();
         }
         g->idx.push_back(j);
+        if (exc) g->lds = std::max(g->lds, exc_lds_bytes(t, bk));
     }
     for (auto& g : groups)
         std::stable_sort(g.idx.begin(), g.idx.end(),
@@ -801,7 +807,6 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
         const QmcSweepJob& x = jobs[j];
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
-        if ((rc = refuse_excitations(x.plan))) return rc;
         if (!x.walkers.pos || !x.walkers.logabs || !x.walkers.sign) return fail(QMC_ERR_ARG, "null walker state");
         if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.sweep.keep && x.sweep.keep_every < 1) return fail(QMC_ERR_ARG, "keep_every must be >= 1");
@@ -818,6 +823,8 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
     SJob* dev = (SJob*)(b->dev + slot * b->slot_bytes);
     struct Launch {
         int rad, bucket, first, n, nwaves;
+        bool exc;
+        size_t lds;
         const SpecEntry* spec;
     };
     std::vector<Launch> launches;
@@ -838,11 +845,16 @@ int qmc_metropolis_sweep_batch(QmcBatch* b, int32_t precision, const QmcSweepJob
             waves += (x.nw + G - 1) / G;
         }
         if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
-        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.spec});
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.exc, g.lds, g.spec});
     }
     hipStream_t s = (hipStream_t)stream;
     QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(SJob), hipMemcpyHostToDevice, s));
     for (const auto& L : launches) {
+        if (L.exc) {
+            if (QMC_BY_RAD(launch_sweep_batch_exc, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, L.lds, precision, s))
+                return fail(QMC_ERR_UNSUPPORTED, kNoExcBucket);
+            continue;
+        }
         if (L.spec && L.spec->sweep_batch(dev + L.first, L.n, L.nwaves, precision, s) == 0) continue;
         const int r = precision == QMC_PREC_FP64
                           ? QMC_BY_RAD(launch_sweep_batch_f64, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, s)
@@ -865,7 +877,6 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
         const QmcEvalJob& x = jobs[j];
         int rc = check_common(x.plan, &x.params, x.nw);
         if (rc) return rc;
-        if ((rc = refuse_excitations(x.plan))) return rc;
         if (!x.pos) return fail(QMC_ERR_ARG, "null pos");
         if (!x.params.entry_w) return fail(QMC_ERR_ARG, "missing parameter pointer entry_w");
         if (x.nw > 0) live.push_back(x);
@@ -881,6 +892,8 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
     WJob* dev = (WJob*)(b->dev + slot * b->slot_bytes);
     struct Launch {
         int rad, bucket, first, n, nwaves;
+        bool exc;
+        size_t lds;
         const SpecEntry* spec;
     };
     std::vector<Launch> launches;
@@ -901,11 +914,17 @@ int qmc_wf_eval_batch(QmcBatch* b, int32_t precision, int32_t mode, const QmcEva
             waves += (x.nw + G - 1) / G;
         }
         if (waves > (int64_t)1 << 30) return fail(QMC_ERR_ARG, "batch too large");
-        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.spec});
+        launches.push_back(Launch{g.rad, g.bucket, first, k - first, (int)waves, g.exc, g.lds, g.spec});
     }
     hipStream_t s = (hipStream_t)stream;
     QMC_HIP(hipMemcpyAsync(dev, host, (size_t)k * sizeof(WJob), hipMemcpyHostToDevice, s));
     for (const auto& L : launches) {
+        if (L.exc) {
+            if (QMC_BY_RAD(launch_wf_batch_exc, L.rad, dev + L.first, L.n, L.nwaves, L.bucket, L.lds, precision, mode,
+                           s))
+                return fail(QMC_ERR_UNSUPPORTED, kNoExcBucket);
+            continue;
+        }
         if (L.spec && L.spec->wf_batch(dev + L.first, L.n, L.nwaves, precision, mode, s) == 0) continue;
         const int r =
             precision == QMC_PREC_FP64

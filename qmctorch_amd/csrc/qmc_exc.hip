@@ -1,6 +1,6 @@
 // Excitation (trace-trick) kernels for plans with more than QMC_MAXCONF unique
-// configurations per spin: psi / local energy and the Metropolis sweep, both
-// precisions, register buckets up to 16.
+// configurations per spin: psi / local energy and the Metropolis sweep (single
+// plans and batched jobs), both precisions, register buckets up to 16.
 #include "qmc_adjoint.h"
 #include "qmc_launch.h"
 
@@ -56,6 +56,60 @@ int launch_sweep_exc<QMC_RAD>(const SArgs& a, int prec, hipStream_t s) {
 #define QMC_CASE(K)                       \
     case K:                               \
         go_sweep<K, QMC_RAD>(a, prec, s); \
+        return 0;
+        QMC_EXC_BUCKETS(QMC_CASE)
+#undef QMC_CASE
+        default: return -1;
+    }
+}
+
+template <int NM, int RAD>
+static void go_sweep_batch(const SJob* jobs, int njobs, int nwaves, size_t lds, int prec, hipStream_t s) {
+    const dim3 g((unsigned)nwaves);
+    if (prec == QMC_PREC_FP64)
+        hipLaunchKernelGGL((sweep_batch_kernel_exc<double, NM, RAD>), g, dim3(64), lds, s, jobs, njobs, nwaves);
+    else
+        hipLaunchKernelGGL((sweep_batch_kernel_exc<float, NM, RAD>), g, dim3(64), lds, s, jobs, njobs, nwaves);
+}
+
+template <>
+int launch_sweep_batch_exc<QMC_RAD>(const SJob* jobs, int njobs, int nwaves, int bucket, size_t lds, int prec,
+                                    hipStream_t s) {
+    switch (bucket) {
+#define QMC_CASE(K)                                                    \
+    case K:                                                            \
+        go_sweep_batch<K, QMC_RAD>(jobs, njobs, nwaves, lds, prec, s); \
+        return 0;
+        QMC_EXC_BUCKETS(QMC_CASE)
+#undef QMC_CASE
+        default: return -1;
+    }
+}
+
+template <int NM, int RAD>
+static void go_wf_batch(const WJob* jobs, int njobs, int nwaves, size_t lds, int prec, int mode, hipStream_t s) {
+    const dim3 g((unsigned)nwaves);
+    if (prec == QMC_PREC_FP64) {
+        if (mode == QMC_EVAL_ELOC)
+            hipLaunchKernelGGL((wf_batch_kernel_exc<double, NM, RAD, true>), g, dim3(64), lds, s, jobs, njobs, nwaves);
+        else
+            hipLaunchKernelGGL((wf_batch_kernel_exc<double, NM, RAD, false>), g, dim3(64), lds, s, jobs, njobs,
+                               nwaves);
+    } else {
+        if (mode == QMC_EVAL_ELOC)
+            hipLaunchKernelGGL((wf_batch_kernel_exc<float, NM, RAD, true>), g, dim3(64), lds, s, jobs, njobs, nwaves);
+        else
+            hipLaunchKernelGGL((wf_batch_kernel_exc<float, NM, RAD, false>), g, dim3(64), lds, s, jobs, njobs, nwaves);
+    }
+}
+
+template <>
+int launch_wf_batch_exc<QMC_RAD>(const WJob* jobs, int njobs, int nwaves, int bucket, size_t lds, int prec, int mode,
+                                 hipStream_t s) {
+    switch (bucket) {
+#define QMC_CASE(K)                                                       \
+    case K:                                                               \
+        go_wf_batch<K, QMC_RAD>(jobs, njobs, nwaves, lds, prec, mode, s); \
         return 0;
         QMC_EXC_BUCKETS(QMC_CASE)
 #undef QMC_CASE

@@ -1313,6 +1313,19 @@ __global__ __launch_bounds__(kBlock) void wf_batch_kernel(const WJob* __restrict
     wf_walkers<T, NM, RAD, ELOC, AO>(a, L);
 }
 
+// wf_batch_kernel for excitation plans: one wavefront per block (as
+// wf_kernel_exc), the launch's dynamic LDS sized for its largest job.
+template <typename T, int NM, int RAD, bool ELOC>
+__global__ __launch_bounds__(64) void wf_batch_kernel_exc(const WJob* __restrict__ jobs, int njobs, int nwaves) {
+    const int wave = global_wave();
+    if (wave >= nwaves) return;
+    const int j = find_job(jobs, njobs, wave);
+    const KArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    wf_walkers<T, NM, RAD, ELOC, AoGeneric, true>(a, L);
+}
+
 struct SArgs {
     Tables t;
     QmcParams p;
@@ -1519,6 +1532,19 @@ struct SJob {
     SArgs a;
     int64_t wave0;
 };
+
+// sweep_batch_kernel for excitation plans: one wavefront per block (as
+// sweep_kernel_exc), the launch's dynamic LDS sized for its largest job.
+template <typename T, int NM, int RAD>
+__global__ __launch_bounds__(64) void sweep_batch_kernel_exc(const SJob* __restrict__ jobs, int njobs, int nwaves) {
+    const int wave = global_wave();
+    if (wave >= nwaves) return;
+    const int j = find_job(jobs, njobs, wave);
+    const SArgs a = jobs[j].a;
+    const WalkerLane L = lane_layout_w(a.t, a.nw, wave - (int)jobs[j].wave0);
+    if ((int64_t)L.wave * L.G >= a.nw) return;
+    sweep_walkers<T, NM, RAD, AoGeneric, false, true>(a, L);
+}
 
 template <typename T, int NM, int RAD, typename AO = AoGeneric>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(NM >= 16 ? 2 : 1))) void sweep_batch_kernel(

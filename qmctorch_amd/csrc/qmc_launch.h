@@ -64,6 +64,14 @@ constexpr size_t kExcEnergyAdjointLds = 64 * 1024;
 // excitation (trace-trick) kernels, any precision; register buckets up to 16
 template <int RAD> int launch_wf_exc(const KArgs& a, int precision, int mode, hipStream_t s);
 template <int RAD> int launch_sweep_exc(const SArgs& a, int precision, hipStream_t s);
+// batched excitation-plan launches: jobs (device) of one register bucket, one
+// wavefront per block, lds = the largest job's exc_lds_bytes
+template <int RAD>
+int launch_sweep_batch_exc(const SJob* jobs, int njobs, int nwaves, int bucket, size_t lds, int precision,
+                           hipStream_t s);
+template <int RAD>
+int launch_wf_batch_exc(const WJob* jobs, int njobs, int nwaves, int bucket, size_t lds, int precision, int mode,
+                        hipStream_t s);
 // HMC / drift: MC = 1 for single-configuration plans, QMC_MAXCONF otherwise
 template <int RAD, int MC> int launch_hmc_f32(const HArgs& a, hipStream_t s);
 template <int RAD, int MC> int launch_hmc_f64(const HArgs& a, hipStream_t s);

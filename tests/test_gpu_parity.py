@@ -427,11 +427,10 @@ def test_solver_run_auto_gradient():
 
 def test_excitation_plans_refuse_entry_points_without_that_path():
     """Plans with more than 8 unique configurations per spin run psi, pdf, the
-    local energy, the sweep, the drift / HMC and both adjoints through the
-    excitation kernels; the per-walker adjoint rows and the batched launches
-    say that they have no such path."""
+    local energy, the sweep, the drift / HMC, both adjoints and the batched
+    launches through the excitation kernels; the per-walker adjoint rows say
+    that they have no such path."""
     _say("case excitation refusals")
-    from qmctorch_amd.serving import BatchRunner, SystemContext, WalkerSet
     from qmctorch_amd.wavefunction.slater_jastrow import wf_adjoint_rows
 
     mol, kw = helpers.CASES["lih_631g_sd46_ee_en"]()
@@ -439,6 +438,3 @@ def test_excitation_plans_refuse_entry_points_without_that_path():
     pos = helpers.positions(mol, 8, seed=2).to(DEV)
     with pytest.raises(RuntimeError, match="excitation"):
         wf_adjoint_rows(wf, pos)
-    ws = WalkerSet(SystemContext(wf, "lih_sd46"), pos.clone(), seed=1)
-    with pytest.raises(RuntimeError, match="excitation"):
-        BatchRunner(max_jobs=4).sweep([(ws, 2)])

@@ -109,6 +109,46 @@ def test_batch_local_energy_equals_single_eval(contexts):
         assert torch.equal(e, ref), w.ctx.name
 
 
+def test_batch_excitation_plans_equal_per_system_launches(contexts):
+    """Jobs of excitation plans (more than 8 unique configurations per spin)
+    run through the batched excitation kernels, beside ordinary jobs: sweeps
+    bit-identical to their per-system launches, local energies identical to
+    the single-plan evaluation."""
+    _say("case batch with excitation plans (LiH and H2O single_double(4,6)) vs per-system launches")
+    exc = {}
+    for case, name in (("lih_631g_sd46_ee_en", "lih_sd46"), ("h2o_ccpvdz_sd46_ee", "h2o_sd46")):
+        mol, kw = helpers.CASES[case]()
+        wf, _ = helpers.make_pair(mol, precision="mixed", **kw)
+        exc[name] = (SystemContext(wf, name), mol)
+    jobs = [("h2o", 200, 21), ("lih_sd46", 150, 22), ("h2o_sd46", 61, 23), ("lih_sd46", 37, 24), ("lih", 90, 25)]
+
+    def sets():
+        out = []
+        for name, nw, seed in jobs:
+            if name in exc:
+                ctx, mol = exc[name]
+                out.append(WalkerSet(ctx, helpers.positions(mol, nw, seed=seed).cuda(), seed))
+            else:
+                ctx = contexts[name]
+                out.append(WalkerSet(ctx, catalog.initial_positions(ctx.wf.mol, nw, seed), seed))
+        return out
+
+    runner = BatchRunner(8)
+    wb, ws = sets(), sets()
+    runner.sweep([(w, 11) for w in wb])
+    runner.sweep([(w, 6) for w in wb])
+    sweep_each([(w, 17) for w in ws])
+    torch.cuda.synchronize()
+    for (name, nw, _), a, b in zip(jobs, wb, ws):
+        assert _same(a, b), name
+        assert int(a.naccept.sum()) > 0, name
+    outs = [torch.empty(w.nw, dtype=torch.float64, device="cuda") for w in wb]
+    runner.local_energy(list(zip(wb, outs)))
+    for (name, _, _), w, e in zip(jobs, wb, outs):
+        ref = w.ctx.wf.local_energy_parts(w.pos)["eloc"]
+        assert torch.equal(e, ref), name
+
+
 def test_batch_local_energy_fp64_vs_oracle():
     _say("case batch fp64 local energy vs oracle (LiH CAS(2,2), H2O)")
     runner = BatchRunner(8)
