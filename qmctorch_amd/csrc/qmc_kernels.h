@@ -49,9 +49,9 @@ struct WalkerLane {
 };
 
 // wave: index of this wavefront among the waves serving the nw walkers
-__device__ __forceinline__ WalkerLane lane_layout_w(const Tables& t, int64_t nw, int wave) {
+__device__ __forceinline__ WalkerLane lane_layout_l(const Tables& t, int64_t nw, int wave, int lane) {
     WalkerLane L;
-    L.lane = threadIdx.x & 63;
+    L.lane = lane;
     L.wave = wave;
     const int Ne = t.nelec;
     L.G = 64 / Ne;
@@ -69,6 +69,10 @@ __device__ __forceinline__ WalkerLane lane_layout_w(const Tables& t, int64_t nw,
     L.n_sig = L.spin == 0 ? t.nup : t.ndown;
     L.gmask = (L.n_sig >= 64) ? ~0ull : (((1ull << L.n_sig) - 1ull) << L.gbase);
     return L;
+}
+
+__device__ __forceinline__ WalkerLane lane_layout_w(const Tables& t, int64_t nw, int wave) {
+    return lane_layout_l(t, nw, wave, threadIdx.x & 63);
 }
 
 __device__ __forceinline__ int global_wave() {
@@ -1399,9 +1403,15 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
     const double sd = sqrt(a.sw.sigma2);
     int nacc = 0;
     for (int s = 0; s < a.sw.nstep; ++s) {
+        // the lane layout formed again each step from an opaque lane index:
+        // its dozen per-lane fields are then not held over the step loop
+        // (which spilled VGPRs to scratch in the H2O kernel)
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));
+        const WalkerLane Ls = lane_layout_l(t, a.nw, L.wave, lane);
         const uint64_t step = (uint64_t)(a.sw.step0 + s);
         // counter: (walker id lo, walker id hi, step, electron | stream<<16)
-        const u4 ctr{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, (uint32_t)L.e};
+        const u4 ctr{(uint32_t)wid, (uint32_t)(wid >> 32), (uint32_t)step, (uint32_t)Ls.e};
         const u4 rnd = philox4x32_10(ctr, k0, k1);
         double nx, ny, nz;
         if (a.sw.uniform_move) {
@@ -1432,18 +1442,18 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
         const uint32_t spare = a.sw.uniform_move
                                    ? rnd.w
                                    : ((rnd.x & 0xFFu) | ((rnd.y & 0xFFu) << 8) | ((rnd.z & 0xFFu) << 16) | (rnd.w << 24));
-        const uint32_t ubits = (uint32_t)__shfl((int)spare, L.wbase);
+        const uint32_t ubits = (uint32_t)__shfl((int)spare, Ls.wbase);
         if (a.sw.one_electron) {
             // move type "one-elec" (metropolis.py:228-250): only electron
             // floor(Ne V / 2^32) of the walker moves, V the spare bits of
             // electron 1's draw (for Ne = 1 the index is 0 whatever V is)
-            const uint32_t vb = (uint32_t)__shfl((int)spare, L.wbase + 1);
+            const uint32_t vb = (uint32_t)__shfl((int)spare, Ls.wbase + 1);
             const int moved = (int)(((uint64_t)vb * (uint32_t)Ne) >> 32);
-            if (L.e != moved) nx = ny = nz = 0.0;
+            if (Ls.e != moved) nx = ny = nz = 0.0;
         }
         const double xn = x + nx, yn = y + ny, zn = z + nz;
         double sgn_n;
-        double lpn = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, L, xn, yn, zn, logc0, sgn_n, wlds);
+        double lpn = eval_logpsi<T, NM, RAD, AO, WLDS, EXC>(t, a.p, Ls, xn, yn, zn, logc0, sgn_n, wlds);
         if (lpn < lp_zero) lpn = lp_eps;
         // accept iff min(1, psi'^2/psi^2) >= U  <=>  2 (lpn - lp) >= log U; the
         // logarithm in the bulk precision (the hardware log2 in the mixed sweep,
@@ -1465,10 +1475,10 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
             sgn = sgn_n;
             nacc++;
         }
-        if (a.sw.keep && L.valid && (int64_t)step >= a.sw.keep_from &&
+        if (a.sw.keep && Ls.valid && (int64_t)step >= a.sw.keep_from &&
             ((int64_t)step - a.sw.keep_from) % a.sw.keep_every == 0) {
             const int64_t ik = ((int64_t)step - a.sw.keep_from) / a.sw.keep_every;
-            double* kp = a.sw.keep + ((size_t)ik * a.nw + L.w) * 3 * Ne + 3 * L.e;
+            double* kp = a.sw.keep + ((size_t)ik * a.nw + Ls.w) * 3 * Ne + 3 * Ls.e;
             kp[0] = x;
             kp[1] = y;
             kp[2] = z;
