@@ -83,23 +83,30 @@ __device__ __forceinline__ double frcp(double x) { return 1.0 / x; }
 // [1/sqrt 2, sqrt 2), z = (m' - 1)/(m' + 1), |z| < 0.172,
 // log m' = 2 atanh z = 2 (z + z^3/3 + ... + z^21/21) (truncation < 1e-19),
 // log m = log m' - [m' = 2m] log 2
+// A wave-uniform constant formed where it is used (scalar moves) rather than
+// hoisted by the compiler over an enclosing loop into registers it then spills.
+__device__ __forceinline__ double sconst(double v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
 __device__ __forceinline__ double log_mantissa(double m) {
     const bool up = m < 0.70710678118654752;
     const double mm = up ? 2.0 * m : m;
     const double z = (mm - 1.0) / (mm + 1.0);
     const double s = z * z;
-    double p = 1.0 / 21.0;
-    p = fma(p, s, 1.0 / 19.0);
-    p = fma(p, s, 1.0 / 17.0);
-    p = fma(p, s, 1.0 / 15.0);
-    p = fma(p, s, 1.0 / 13.0);
-    p = fma(p, s, 1.0 / 11.0);
-    p = fma(p, s, 1.0 / 9.0);
-    p = fma(p, s, 1.0 / 7.0);
-    p = fma(p, s, 1.0 / 5.0);
-    p = fma(p, s, 1.0 / 3.0);
+    double p = sconst(1.0 / 21.0);
+    p = fma(p, s, sconst(1.0 / 19.0));
+    p = fma(p, s, sconst(1.0 / 17.0));
+    p = fma(p, s, sconst(1.0 / 15.0));
+    p = fma(p, s, sconst(1.0 / 13.0));
+    p = fma(p, s, sconst(1.0 / 11.0));
+    p = fma(p, s, sconst(1.0 / 9.0));
+    p = fma(p, s, sconst(1.0 / 7.0));
+    p = fma(p, s, sconst(1.0 / 5.0));
+    p = fma(p, s, sconst(1.0 / 3.0));
     p = fma(p, s, 1.0);
-    return 2.0 * z * p - (up ? 0.6931471805599453 : 0.0);
+    return 2.0 * z * p - (up ? sconst(0.6931471805599453) : 0.0);
 }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }  // hardware, 1 ulp
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }

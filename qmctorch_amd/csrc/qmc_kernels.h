@@ -55,7 +55,10 @@ __device__ __forceinline__ WalkerLane lane_layout_l(const Tables& t, int64_t nw,
     L.wave = wave;
     const int Ne = t.nelec;
     L.G = 64 / Ne;
-    L.wl = L.lane / Ne;
+    // floor(lane / Ne) as a multiply and shift: (lane ceil(2^16 / Ne)) >> 16 is
+    // exact for lane < 64 and every Ne <= 64 (the layout is formed every step)
+    const uint32_t inv = (65536u + (uint32_t)Ne - 1u) / (uint32_t)Ne;
+    L.wl = (int)(((uint32_t)L.lane * inv) >> 16);
     L.lane_real = L.wl < L.G;
     L.wbase = L.lane_real ? L.wl * Ne : 64 - Ne;
     L.e = L.lane - L.wbase;
@@ -1484,14 +1487,19 @@ __device__ __forceinline__ void sweep_walkers(const SArgs& a, const WalkerLane& 
             kp[2] = z;
         }
     }
-    if (L.valid) {
-        pp[0] = x;
-        pp[1] = y;
-        pp[2] = z;
-        if (L.e == 0) {
-            a.wk.logabs[L.w] = lp;
-            a.wk.sign[L.w] = sgn;
-            if (a.wk.naccept) a.wk.naccept[L.w] += nacc;
+    // the layout (and the position pointer) formed again for the stores
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const WalkerLane Le = lane_layout_l(t, a.nw, L.wave, lane);
+    if (Le.valid) {
+        double* pe = a.wk.pos + (size_t)Le.wclamp * 3 * Ne + 3 * Le.e;
+        pe[0] = x;
+        pe[1] = y;
+        pe[2] = z;
+        if (Le.e == 0) {
+            a.wk.logabs[Le.w] = lp;
+            a.wk.sign[Le.w] = sgn;
+            if (a.wk.naccept) a.wk.naccept[Le.w] += nacc;
         }
     }
 }
